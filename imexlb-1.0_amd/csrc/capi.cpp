@@ -1,0 +1,812 @@
+// capi.cpp -- the C ABI (include/imexlb.h): context, device memory, and the
+// phase state machine that maps the reference's four per-step procedures
+// (main.f90:57-61) onto the gfx950 kernels.
+//
+// Fused mode (default): Collision of step t+1, Propagation and
+// PostProcessing of step t run as one pull kernel launched from
+// PostProcessing; the post-collision populations ping-pong between two
+// buffers.  f, p and u of step t are not written by that kernel; when a
+// caller reads them (Monitor, downloads) one stream+moments pass rebuilds
+// them from the buffer the fused kernel streamed from, which stays intact
+// until the next PostProcessing.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/imexlb.h"
+#include "geometry.hpp"
+#include "kernels.hpp"
+
+using namespace imexlb;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(e_ == hipErrorOutOfMemory ? IMEXLB_E_OOM : IMEXLB_E_HIP,                       \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                            \
+    } while (0)
+
+enum Stage { S_NONE, S_PRE, S_COLLIDED, S_FILLED, S_PENDING, S_STREAMED, S_FUSED };
+
+struct TimedLaunch {
+    int cls;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct imexlb_ctx {
+    int dim = 2, Q = 9, device = -1, prec = 0;
+    bool host_only = true;
+    RefGeometry g;
+    BcTable bc;
+    bool geom_set = false;
+    double ome = 0.0, rho0 = 1.0, uu = 0.0;
+
+    // device layout
+    int64_t Px = 0, Py = 0, Pz = 1, PxPy = 0, o0 = 0, plane = 0;
+    size_t ssz = 8;
+    KParams kp{};
+    uint32_t skip_c = 0xffffffffu;
+
+    hipStream_t stream = nullptr;
+    void *fb[2] = {nullptr, nullptr};
+    void *f = nullptr;
+    double *p = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr, *rv = nullptr;
+    uint8_t *geo = nullptr;
+    int64_t *fill_dst = nullptr, *fill_src = nullptr;
+    int64_t nfill = 0;
+    unsigned long long *red = nullptr;
+    size_t dev_bytes = 0;
+
+    // phase state
+    Stage stage = S_NONE;
+    int cur = 0;
+    bool fused = true;
+    bool have_pu = false, have_f = false, have_fb = false;
+    int lazy_src = -1;
+
+    // profiling
+    bool prof = false;
+    std::vector<TimedLaunch> timed;
+    int64_t prof_n[3] = {0, 0, 0};
+    double prof_ms[3] = {0, 0, 0};
+
+    int64_t pidx(int x, int y, int z) const { return o0 + x + Px * y + PxPy * z; }
+
+    StepArgs args() const
+    {
+        StepArgs a;
+        a.kp = kp;
+        a.p = p;
+        a.ux = ux;
+        a.uy = uy;
+        a.uz = uz;
+        a.geo = geo;
+        return a;
+    }
+
+    int begin(int cls, hipEvent_t &b)
+    {
+        (void)cls;
+        if (!prof) return IMEXLB_OK;
+        HIP_TRY(hipEventCreate(&b));
+        HIP_TRY(hipEventRecord(b, stream));
+        return IMEXLB_OK;
+    }
+    int end(int cls, hipEvent_t b)
+    {
+        if (!prof) return IMEXLB_OK;
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        HIP_TRY(hipEventRecord(e, stream));
+        timed.push_back(TimedLaunch{cls, b, e});
+        return IMEXLB_OK;
+    }
+
+    template <class F>
+    int launch(int cls, F &&fn)
+    {
+        hipEvent_t b = nullptr;
+        int rc = begin(cls, b);
+        if (rc) return rc;
+        hipError_t e = fn();
+        if (e != hipSuccess) return fail(IMEXLB_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+        return end(cls, b);
+    }
+
+    int run_fill(int buf)
+    {
+        return launch(1, [&] { return launch_fill(prec, fb[buf], fill_dst, fill_src, nfill, stream); });
+    }
+
+    int run_step(int mode, const void *src, void *dst, void *fout)
+    {
+        StepArgs a = args();
+        a.src = src;
+        a.dst = dst;
+        a.fout = fout;
+        return launch(mode == K_COLLIDE ? 0 : 2, [&] { return launch_step(dim, prec, mode, a, stream); });
+    }
+
+    // rebuild f, p, u of the current pre-collision state if the fused
+    // kernel skipped writing them
+    int materialize()
+    {
+        if (have_pu && have_f) return IMEXLB_OK;
+        if (lazy_src < 0) {
+            if (have_pu) return IMEXLB_OK;
+            return fail(IMEXLB_E_STATE, "fields requested before InitUP/upload");
+        }
+        int rc = run_step(K_WRITE_F | K_WRITE_PU, fb[lazy_src], nullptr, f);
+        if (rc) return rc;
+        have_pu = have_f = true;
+        return IMEXLB_OK;
+    }
+
+    void release()
+    {
+        if (host_only) return;
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &t : timed) {
+            (void)hipEventDestroy(t.a);
+            (void)hipEventDestroy(t.b);
+        }
+        timed.clear();
+        void *ptrs[] = {fb[0], fb[1], f, p, ux, uy, uz, rv, geo, fill_dst, fill_src, red};
+        for (void *q : ptrs)
+            if (q) (void)hipFree(q);
+        fb[0] = fb[1] = f = nullptr;
+        p = ux = uy = uz = rv = nullptr;
+        geo = nullptr;
+        fill_dst = fill_src = nullptr;
+        red = nullptr;
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+};
+
+namespace {
+
+int dev_alloc(imexlb_ctx *c, void **ptr, size_t bytes)
+{
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return fail(e == hipErrorOutOfMemory ? IMEXLB_E_OOM : IMEXLB_E_HIP,
+                    "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    c->dev_bytes += bytes;
+    e = hipMemset(*ptr, 0, bytes);
+    if (e != hipSuccess) return fail(IMEXLB_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    return IMEXLB_OK;
+}
+
+#define CHECK_CTX(c)                                                       \
+    do {                                                                   \
+        if (!(c)) return fail(IMEXLB_E_ARG, "null context");              \
+    } while (0)
+#define CHECK_DEV(c)                                                                   \
+    do {                                                                               \
+        CHECK_CTX(c);                                                                  \
+        if ((c)->host_only) return fail(IMEXLB_E_NODEV, "host-only context");         \
+        HIP_TRY(hipSetDevice((c)->device));                                            \
+    } while (0)
+
+size_t storage_bytes(int prec) { return prec == 0 ? 8 : prec == 1 ? 4 : 2; }
+
+const std::vector<int32_t> *pick_list(const imexlb_ctx *c, int list)
+{
+    const RefGeometry &g = c->g;
+    switch (list) {
+    case IMEXLB_LIST_FLUID: return &g.fluid_id;
+    case IMEXLB_LIST_BL: return &g.bl;
+    case IMEXLB_LIST_BR: return &g.br;
+    case IMEXLB_LIST_BU: return &g.bu;
+    case IMEXLB_LIST_BD: return &g.bd;
+    case IMEXLB_LIST_BF: return &g.bf;
+    case IMEXLB_LIST_BB: return &g.bb;
+    case IMEXLB_LIST_USER: return &g.b_user;
+    default: return nullptr;
+    }
+}
+
+// host staging helpers: interior node (x,y,z) <-> reference id / dense index
+template <class Fn>
+static void for_interior(const imexlb_ctx *c, Fn fn)
+{
+    const int nx = c->kp.nx, ny = c->kp.ny, nz = c->dim == 3 ? c->kp.nz : 0;
+    int64_t k = 0;
+    for (int z = 0; z <= nz; ++z)
+        for (int y = 0; y <= ny; ++y)
+            for (int x = 0; x <= nx; ++x, ++k) fn(x, y, z, k, c->g.id(x, y, z), c->pidx(x, y, z));
+}
+
+double load_value(const uint8_t *base, size_t i, int prec)
+{
+    switch (prec) {
+    case 0: {
+        double v;
+        std::memcpy(&v, base + i * 8, 8);
+        return v;
+    }
+    case 1: {
+        float v;
+        std::memcpy(&v, base + i * 4, 4);
+        return double(v);
+    }
+    case 2: {
+        _Float16 v;
+        std::memcpy(&v, base + i * 2, 2);
+        return double(float(v));
+    }
+    default: {
+        uint16_t b;
+        std::memcpy(&b, base + i * 2, 2);
+        uint32_t w = uint32_t(b) << 16;
+        float v;
+        std::memcpy(&v, &w, 4);
+        return double(v);
+    }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *imexlb_strerror(int code)
+{
+    switch (code) {
+    case IMEXLB_OK: return "success";
+    case IMEXLB_E_ARG: return "invalid argument";
+    case IMEXLB_E_HIP: return "HIP runtime error";
+    case IMEXLB_E_OOM: return "out of memory";
+    case IMEXLB_E_STATE: return "call out of the Collision..PostProcessing order";
+    case IMEXLB_E_UB: return "result depends on a read past the end of the reference arrays";
+    case IMEXLB_E_NODEV: return "compute call on a host-only context";
+    default: return "unknown error";
+    }
+}
+
+const char *imexlb_last_error(void) { return g_last_error.c_str(); }
+
+int imexlb_device_count(int *n)
+{
+    if (!n) return fail(IMEXLB_E_ARG, "null pointer");
+    int k = 0;
+    hipError_t e = hipGetDeviceCount(&k);
+    *n = e == hipSuccess ? k : 0;
+    return IMEXLB_OK;
+}
+
+int imexlb_create(imexlb_ctx **out, int dim, int nx, int ny, int nz, int device, int precision)
+{
+    if (!out) return fail(IMEXLB_E_ARG, "null output pointer");
+    *out = nullptr;
+    if (dim != 2 && dim != 3) return fail(IMEXLB_E_ARG, "dim must be 2 or 3");
+    if (nx < 2 || ny < 2 || (dim == 3 && nz < 2))
+        return fail(IMEXLB_E_ARG, "every axis needs at least 3 nodes (cart_mpi.f90:107-110)");
+    if (precision < IMEXLB_F64 || precision > IMEXLB_BF16_F32) return fail(IMEXLB_E_ARG, "unknown precision");
+    std::unique_ptr<imexlb_ctx> c(new (std::nothrow) imexlb_ctx);
+    if (!c) return fail(IMEXLB_E_OOM, "context");
+    c->dim = dim;
+    c->Q = dim == 2 ? 9 : 27;
+    c->prec = precision;
+    c->ssz = storage_bytes(precision);
+    c->g.init(dim, nx, ny, dim == 3 ? nz : 0);
+    if (c->g.A >= (int64_t(1) << 31)) return fail(IMEXLB_E_ARG, "array_size exceeds the reference's 32-bit ids");
+
+    // padded device layout, one ghost layer
+    c->Px = int64_t(nx) + 3;
+    c->Py = int64_t(ny) + 3;
+    c->Pz = dim == 3 ? int64_t(nz) + 3 : 1;
+    c->PxPy = c->Px * c->Py;
+    c->o0 = 1 + c->Px + (dim == 3 ? c->PxPy : 0);
+    c->plane = ((c->Px * c->Py * c->Pz + 63) / 64) * 64;
+    const int64_t nodes = int64_t(nx + 1) * (ny + 1) * (dim == 3 ? nz + 1 : 1);
+    if (c->plane >= (int64_t(1) << 31) || nodes >= (int64_t(1) << 31))
+        return fail(IMEXLB_E_ARG, "grid too large for 32-bit node indexing");
+
+    KParams &k = c->kp;
+    k.nx = nx;
+    k.ny = ny;
+    k.nz = dim == 3 ? nz : 0;
+    k.N = uint32_t(nodes);
+    k.div_x.init(uint32_t(nx + 1));
+    k.div_y.init(uint32_t(ny + 1));
+    // host self-check of the multiply-high division on the row/plane ends
+    const uint32_t probes[] = {0u, uint32_t(nx), uint32_t(nx + 1), k.N - 1u, k.N / 2u, 0xfffffffeu};
+    for (uint32_t n : probes) {
+        if (k.div_x.host_div(n) != n / uint32_t(nx + 1) || k.div_y.host_div(n) != n / uint32_t(ny + 1))
+            return fail(IMEXLB_E_ARG, "fast division self-check failed");
+    }
+    k.o0 = c->o0;
+    k.Px = c->Px;
+    k.PxPy = dim == 3 ? c->PxPy : 0;
+    k.plane = c->plane;
+    k.rdy = c->g.dy;
+    k.rdz = c->g.dz;
+
+    if (device >= 0) {
+        int ndev = 0;
+        HIP_TRY(hipGetDeviceCount(&ndev));
+        if (device >= ndev) return fail(IMEXLB_E_ARG, "device index out of range");
+        c->device = device;
+        c->host_only = false;
+        HIP_TRY(hipSetDevice(device));
+        HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        const size_t pop = size_t(c->Q) * size_t(c->plane) * c->ssz;
+        const size_t fld = size_t(c->plane) * sizeof(double);
+        int rc;
+        if ((rc = dev_alloc(c.get(), &c->fb[0], pop)) || (rc = dev_alloc(c.get(), &c->fb[1], pop)) ||
+            (rc = dev_alloc(c.get(), &c->f, pop)) || (rc = dev_alloc(c.get(), (void **)&c->p, fld)) ||
+            (rc = dev_alloc(c.get(), (void **)&c->ux, fld)) || (rc = dev_alloc(c.get(), (void **)&c->uy, fld)) ||
+            (dim == 3 && (rc = dev_alloc(c.get(), (void **)&c->uz, fld))) ||
+            (rc = dev_alloc(c.get(), (void **)&c->geo, size_t(c->plane))) ||
+            (rc = dev_alloc(c.get(), (void **)&c->rv, 100 * sizeof(double))) ||
+            (rc = dev_alloc(c.get(), (void **)&c->red, 64))) {
+            c->release();
+            return rc;
+        }
+    }
+    *out = c.release();
+    return IMEXLB_OK;
+}
+
+int imexlb_destroy(imexlb_ctx *c)
+{
+    if (!c) return IMEXLB_OK;
+    c->release();
+    delete c;
+    return IMEXLB_OK;
+}
+
+int imexlb_set_params(imexlb_ctx *c, double ome, double rho0, double uu)
+{
+    CHECK_CTX(c);
+    if (!std::isfinite(ome) || !std::isfinite(rho0) || !std::isfinite(uu) || rho0 == 0.0)
+        return fail(IMEXLB_E_ARG, "non-finite parameter or rho0 == 0");
+    c->ome = c->kp.ome = ome;
+    c->rho0 = c->kp.rho0 = rho0;
+    c->uu = c->kp.uu = uu;
+    return IMEXLB_OK;
+}
+
+int imexlb_set_geometry(imexlb_ctx *c, double pos, double radius)
+{
+    CHECK_CTX(c);
+    if (!std::isfinite(pos) || !std::isfinite(radius)) return fail(IMEXLB_E_ARG, "non-finite geometry");
+    c->g.build(pos, radius);
+    c->bc = resolve_boundary(c->g);
+    c->geom_set = true;
+    if (c->bc.undefined) return fail(IMEXLB_E_UB, c->bc.error);
+    if (c->host_only) return IMEXLB_OK;
+
+    HIP_TRY(hipSetDevice(c->device));
+    // device mask: 1 on interior fluid nodes
+    std::vector<uint8_t> geo(size_t(c->plane), 0);
+    const RefGeometry &g = c->g;
+    for (int32_t id : g.fluid_id) {
+        int x, y, z;
+        g.coords(id, x, y, z);
+        geo[size_t(c->pidx(x, y, z))] = 1;
+    }
+    if (!g.fluid_id.empty()) {
+        int x, y, z;
+        g.coords(g.fluid_id[0], x, y, z);
+        c->skip_c = uint32_t(c->pidx(x, y, z));
+    }
+    HIP_TRY(hipMemcpy(c->geo, geo.data(), geo.size(), hipMemcpyHostToDevice));
+
+    // resolved boundary table in device slots
+    std::vector<int64_t> dst, src;
+    dst.reserve(c->bc.entries.size());
+    src.reserve(c->bc.entries.size());
+    for (const FillEntry &e : c->bc.entries) {
+        int x, y, z;
+        const int64_t cell = e.ref_slot / c->Q;
+        const int q = int(e.ref_slot % c->Q);
+        g.coords(cell, x, y, z);
+        dst.push_back(int64_t(q) * c->plane + c->pidx(x, y, z));
+        if (e.ref_src < 0) {
+            src.push_back(-1);
+        } else {
+            const int64_t sc = e.ref_src / c->Q;
+            const int sq = int(e.ref_src % c->Q);
+            g.coords(sc, x, y, z);
+            src.push_back(int64_t(sq) * c->plane + c->pidx(x, y, z));
+        }
+    }
+    if (c->fill_dst) (void)hipFree(c->fill_dst);
+    if (c->fill_src) (void)hipFree(c->fill_src);
+    c->fill_dst = c->fill_src = nullptr;
+    c->nfill = int64_t(dst.size());
+    if (c->nfill) {
+        int rc;
+        if ((rc = dev_alloc(c, (void **)&c->fill_dst, dst.size() * 8)) ||
+            (rc = dev_alloc(c, (void **)&c->fill_src, src.size() * 8)))
+            return rc;
+        HIP_TRY(hipMemcpy(c->fill_dst, dst.data(), dst.size() * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->fill_src, src.data(), src.size() * 8, hipMemcpyHostToDevice));
+    }
+    c->stage = S_NONE;
+    c->have_pu = c->have_f = c->have_fb = false;
+    c->lazy_src = -1;
+    return IMEXLB_OK;
+}
+
+
+int imexlb_geometry_size(imexlb_ctx *c, int list, int64_t *n)
+{
+    CHECK_CTX(c);
+    if (!n) return fail(IMEXLB_E_ARG, "null pointer");
+    if (!c->geom_set) return fail(IMEXLB_E_STATE, "SetGeometry not called");
+    if (list == -1) {  // array_size
+        *n = c->g.A;
+        return IMEXLB_OK;
+    }
+    const auto *v = pick_list(c, list);
+    if (!v) return fail(IMEXLB_E_ARG, "unknown list");
+    *n = int64_t(v->size());
+    return IMEXLB_OK;
+}
+
+int imexlb_geometry_list(imexlb_ctx *c, int list, int32_t *out, int64_t cap)
+{
+    CHECK_CTX(c);
+    if (!c->geom_set) return fail(IMEXLB_E_STATE, "SetGeometry not called");
+    const auto *v = pick_list(c, list);
+    if (!v) return fail(IMEXLB_E_ARG, "unknown list");
+    if (cap < int64_t(v->size()) || (!out && !v->empty())) return fail(IMEXLB_E_ARG, "output too small");
+    if (!v->empty()) std::memcpy(out, v->data(), v->size() * sizeof(int32_t));
+    return IMEXLB_OK;
+}
+
+int imexlb_geometry_geo(imexlb_ctx *c, int32_t *geo, int64_t cap)
+{
+    CHECK_CTX(c);
+    if (!c->geom_set) return fail(IMEXLB_E_STATE, "SetGeometry not called");
+    if (!geo || cap < c->g.A) return fail(IMEXLB_E_ARG, "output too small");
+    std::memcpy(geo, c->g.geo.data(), size_t(c->g.A) * sizeof(int32_t));
+    return IMEXLB_OK;
+}
+
+int imexlb_bc_table(imexlb_ctx *c, int64_t *slot, int64_t *src, int64_t cap, int64_t stats[2])
+{
+    CHECK_CTX(c);
+    if (!c->geom_set) return fail(IMEXLB_E_STATE, "SetGeometry not called");
+    const int64_t n = int64_t(c->bc.entries.size());
+    if (stats) {
+        stats[0] = n;
+        stats[1] = c->bc.order_sensitive;
+    }
+    if (slot || src) {
+        if (cap < n) return fail(IMEXLB_E_ARG, "output too small");
+        for (int64_t i = 0; i < n; ++i) {
+            if (slot) slot[i] = c->bc.entries[size_t(i)].ref_slot;
+            if (src) src[i] = c->bc.entries[size_t(i)].ref_src;
+        }
+    }
+    return IMEXLB_OK;
+}
+
+int imexlb_init_up(imexlb_ctx *c, const double *r_vec)
+{
+    CHECK_DEV(c);
+    if (!r_vec) return fail(IMEXLB_E_ARG, "null r_vec");
+    if (!c->geom_set) return fail(IMEXLB_E_STATE, "SetGeometry before InitUP");
+    HIP_TRY(hipMemcpyAsync(c->rv, r_vec, 100 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    StepArgs a = c->args();
+    int rc = c->launch(2, [&] { return launch_init_up(c->dim, a, c->rv, c->stream); });
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));  // r_vec is the caller's
+    c->have_pu = true;
+    c->have_f = c->have_fb = false;
+    c->lazy_src = -1;
+    c->stage = S_NONE;
+    return IMEXLB_OK;
+}
+
+
+int imexlb_upload_fields(imexlb_ctx *c, const double *p, const double *u, int layout)
+{
+    CHECK_DEV(c);
+    if (!p || !u) return fail(IMEXLB_E_ARG, "null field");
+    if (layout != IMEXLB_LAYOUT_REF && layout != IMEXLB_LAYOUT_INTERIOR) return fail(IMEXLB_E_ARG, "layout");
+    if (!c->geom_set) return fail(IMEXLB_E_STATE, "SetGeometry before upload");
+    const int dim = c->dim;
+    const int64_t N = c->kp.N;
+    std::vector<double> hp(size_t(c->plane), 0.0), hu[3];
+    for (int d = 0; d < dim; ++d) hu[d].assign(size_t(c->plane), 0.0);
+    for_interior(c, [&](int, int, int, int64_t k, int64_t rid, int64_t pi) {
+        if (layout == IMEXLB_LAYOUT_REF) {
+            hp[size_t(pi)] = p[rid];
+            for (int d = 0; d < dim; ++d) hu[d][size_t(pi)] = u[rid * dim + d];
+        } else {
+            hp[size_t(pi)] = p[k];
+            for (int d = 0; d < dim; ++d) hu[d][size_t(pi)] = u[int64_t(d) * N + k];
+        }
+    });
+    const size_t bytes = size_t(c->plane) * sizeof(double);
+    HIP_TRY(hipMemcpy(c->p, hp.data(), bytes, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->ux, hu[0].data(), bytes, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->uy, hu[1].data(), bytes, hipMemcpyHostToDevice));
+    if (dim == 3) HIP_TRY(hipMemcpy(c->uz, hu[2].data(), bytes, hipMemcpyHostToDevice));
+    c->have_pu = true;
+    c->have_f = c->have_fb = false;
+    c->lazy_src = -1;
+    c->stage = S_NONE;
+    return IMEXLB_OK;
+}
+
+int imexlb_init_pdf(imexlb_ctx *c)
+{
+    CHECK_DEV(c);
+    if (!c->have_pu || c->lazy_src >= 0) {
+        int rc = c->materialize();
+        if (rc) return rc;
+    }
+    StepArgs a = c->args();
+    a.fout = c->f;
+    int rc = c->launch(2, [&] { return launch_init_pdf(c->dim, c->prec, a, c->stream); });
+    if (rc) return rc;
+    c->have_f = true;
+    c->have_fb = false;
+    c->lazy_src = -1;
+    c->stage = S_PRE;
+    return IMEXLB_OK;
+}
+
+int imexlb_collision(imexlb_ctx *c)
+{
+    CHECK_DEV(c);
+    if (c->stage == S_FUSED) {  // the fused kernel already relaxed this step
+        c->cur = 1 - c->cur;
+        c->stage = S_COLLIDED;
+        return IMEXLB_OK;
+    }
+    if (c->stage != S_PRE) return fail(IMEXLB_E_STATE, "Collision needs InitPDF or a completed step");
+    StepArgs a = c->args();
+    a.src = c->f;
+    a.dst = c->fb[c->cur];
+    int rc = c->launch(2, [&] { return launch_collide(c->dim, c->prec, a, c->stream); });
+    if (rc) return rc;
+    c->have_fb = true;
+    c->stage = S_COLLIDED;
+    return IMEXLB_OK;
+}
+
+int imexlb_boundary_condition(imexlb_ctx *c)
+{
+    CHECK_DEV(c);
+    if (c->stage != S_COLLIDED) return fail(IMEXLB_E_STATE, "BoundaryCondition must follow Collision");
+    int rc = c->run_fill(c->cur);
+    if (rc) return rc;
+    c->stage = S_FILLED;
+    return IMEXLB_OK;
+}
+
+int imexlb_propagation(imexlb_ctx *c)
+{
+    CHECK_DEV(c);
+    if (c->stage != S_FILLED) return fail(IMEXLB_E_STATE, "Propagation must follow BoundaryCondition");
+    if (c->fused) {
+        c->stage = S_PENDING;
+        return IMEXLB_OK;
+    }
+    int rc = c->run_step(K_WRITE_F, c->fb[c->cur], nullptr, c->f);
+    if (rc) return rc;
+    c->have_f = true;
+    c->lazy_src = -1;
+    c->stage = S_STREAMED;
+    return IMEXLB_OK;
+}
+
+int imexlb_post_processing(imexlb_ctx *c)
+{
+    CHECK_DEV(c);
+    if (c->stage == S_PENDING) {
+        int rc = c->run_step(K_COLLIDE, c->fb[c->cur], c->fb[1 - c->cur], nullptr);
+        if (rc) return rc;
+        c->lazy_src = c->cur;
+        c->have_f = c->have_pu = false;
+        c->stage = S_FUSED;
+        return IMEXLB_OK;
+    }
+    if (c->stage != S_STREAMED) return fail(IMEXLB_E_STATE, "PostProcessing must follow Propagation");
+    int rc = c->run_step(K_NOSHIFT | K_WRITE_PU, c->f, nullptr, nullptr);
+    if (rc) return rc;
+    c->have_pu = true;
+    c->stage = S_PRE;
+    return IMEXLB_OK;
+}
+
+int imexlb_set_fused(imexlb_ctx *c, int fused)
+{
+    CHECK_CTX(c);
+    if (c->stage == S_PENDING || c->stage == S_STREAMED)
+        return fail(IMEXLB_E_STATE, "cannot switch between Propagation and PostProcessing");
+    c->fused = fused != 0;
+    return IMEXLB_OK;
+}
+
+int imexlb_advance(imexlb_ctx *c, int64_t nsteps)
+{
+    CHECK_DEV(c);
+    if (nsteps < 0) return fail(IMEXLB_E_ARG, "negative step count");
+    for (int64_t i = 0; i < nsteps; ++i) {
+        int rc;
+        if ((rc = imexlb_collision(c)) || (rc = imexlb_boundary_condition(c)) || (rc = imexlb_propagation(c)) ||
+            (rc = imexlb_post_processing(c)))
+            return rc;
+    }
+    return IMEXLB_OK;
+}
+
+int imexlb_monitor(imexlb_ctx *c, double *um)
+{
+    CHECK_DEV(c);
+    if (!um) return fail(IMEXLB_E_ARG, "null pointer");
+    int rc = c->materialize();
+    if (rc) return rc;
+    StepArgs a = c->args();
+    rc = c->launch(2, [&] { return launch_monitor(c->dim, a, c->skip_c, c->red, c->stream); });
+    if (rc) return rc;
+    unsigned long long bits = 0;
+    HIP_TRY(hipMemcpyAsync(&bits, c->red, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double m;
+    std::memcpy(&m, &bits, sizeof(m));
+    *um = std::sqrt(m);
+    return IMEXLB_OK;
+}
+
+int imexlb_download_fields(imexlb_ctx *c, double *p, double *u, int layout)
+{
+    CHECK_DEV(c);
+    if (layout != IMEXLB_LAYOUT_REF && layout != IMEXLB_LAYOUT_INTERIOR) return fail(IMEXLB_E_ARG, "layout");
+    int rc = c->materialize();
+    if (rc) return rc;
+    const int dim = c->dim;
+    const size_t bytes = size_t(c->plane) * sizeof(double);
+    std::vector<double> h(size_t(c->plane));
+    const int64_t N = c->kp.N;
+    double *srcs[4] = {c->p, c->ux, c->uy, c->uz};
+    for (int k = 0; k <= dim; ++k) {
+        double *out = k == 0 ? p : u;
+        if (!out) continue;
+        HIP_TRY(hipMemcpyAsync(h.data(), srcs[k], bytes, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for_interior(c, [&](int, int, int, int64_t n, int64_t rid, int64_t pi) {
+            const double v = h[size_t(pi)];
+            if (layout == IMEXLB_LAYOUT_REF)
+                out[k == 0 ? rid : rid * dim + (k - 1)] = v;
+            else
+                out[k == 0 ? n : int64_t(k - 1) * N + n] = v;
+        });
+    }
+    return IMEXLB_OK;
+}
+
+
+int imexlb_download_pdf(imexlb_ctx *c, int which, double *out, int layout)
+{
+    CHECK_DEV(c);
+    if (!out) return fail(IMEXLB_E_ARG, "null output");
+    if (layout != IMEXLB_LAYOUT_REF && layout != IMEXLB_LAYOUT_INTERIOR) return fail(IMEXLB_E_ARG, "layout");
+    const void *src;
+    if (which == IMEXLB_PDF_F) {
+        if (c->stage == S_NONE) return fail(IMEXLB_E_STATE, "no populations yet");
+        int rc = c->materialize();
+        if (rc) return rc;
+        src = c->f;
+    } else if (which == IMEXLB_PDF_FB) {
+        if (!c->have_fb) return fail(IMEXLB_E_STATE, "no post-collision state yet");
+        src = c->stage == S_FUSED ? c->fb[c->lazy_src] : c->fb[c->cur];
+    } else {
+        return fail(IMEXLB_E_ARG, "which");
+    }
+    const int Q = c->Q;
+    const int64_t N = c->kp.N;
+    std::vector<uint8_t> h(size_t(c->plane) * c->ssz);
+    for (int q = 0; q < Q; ++q) {
+        HIP_TRY(hipMemcpyAsync(h.data(), (const uint8_t *)src + size_t(q) * c->plane * c->ssz, h.size(),
+                               hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for_interior(c, [&](int, int, int, int64_t n, int64_t rid, int64_t pi) {
+            const double v = load_value(h.data(), size_t(pi), c->prec);
+            if (layout == IMEXLB_LAYOUT_REF)
+                out[rid * Q + q] = v;
+            else
+                out[int64_t(q) * N + n] = v;
+        });
+        if (which == IMEXLB_PDF_FB && layout == IMEXLB_LAYOUT_REF) {
+            for (const FillEntry &e : c->bc.entries) {
+                if (int(e.ref_slot % Q) != q) continue;
+                const int64_t cell = e.ref_slot / Q;
+                int x, y, z;
+                c->g.coords(cell, x, y, z);
+                out[e.ref_slot] = load_value(h.data(), size_t(c->pidx(x, y, z)), c->prec);
+            }
+        }
+    }
+    return IMEXLB_OK;
+}
+
+int imexlb_synchronize(imexlb_ctx *c)
+{
+    CHECK_DEV(c);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return IMEXLB_OK;
+}
+
+int imexlb_profile(imexlb_ctx *c, int enable)
+{
+    CHECK_DEV(c);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (auto &t : c->timed) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    c->timed.clear();
+    for (int k = 0; k < 3; ++k) {
+        c->prof_n[k] = 0;
+        c->prof_ms[k] = 0.0;
+    }
+    c->prof = enable != 0;
+    return IMEXLB_OK;
+}
+
+int imexlb_profile_read(imexlb_ctx *c, int64_t launches[3], double ms[3])
+{
+    CHECK_DEV(c);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (auto &t : c->timed) {
+        float v = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&v, t.a, t.b));
+        c->prof_n[t.cls] += 1;
+        c->prof_ms[t.cls] += double(v);
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    c->timed.clear();
+    for (int k = 0; k < 3; ++k) {
+        if (launches) launches[k] = c->prof_n[k];
+        if (ms) ms[k] = c->prof_ms[k];
+    }
+    return IMEXLB_OK;
+}
+
+int imexlb_info(imexlb_ctx *c, int64_t out[6])
+{
+    CHECK_CTX(c);
+    if (!out) return fail(IMEXLB_E_ARG, "null pointer");
+    out[0] = int64_t(c->kp.N);
+    out[1] = int64_t(c->g.fluid_id.size());
+    out[2] = int64_t(c->plane) * int64_t(c->ssz);
+    out[3] = int64_t(c->dev_bytes);
+    out[4] = int64_t(c->ssz);
+    out[5] = int64_t(c->bc.entries.size());
+    return IMEXLB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,60 @@
+// kernels.hpp -- launch interface between the context (capi.cpp) and the
+// gfx950 kernels (lbm_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace imexlb {
+
+// n / d for 32-bit n by multiply-high and shift (round-up method):
+// s = ceil(log2 d), m = floor(2^32 (2^s - d) / d) + 1, q = (mulhi(n,m) + n) >> s.
+struct FastDiv {
+    uint32_t m = 1, s = 0;
+    void init(uint32_t d)
+    {
+        s = 0;
+        while ((uint64_t(1) << s) < d) ++s;
+        m = uint32_t(((uint64_t(1) << 32) * ((uint64_t(1) << s) - d)) / d + 1);
+    }
+    uint32_t host_div(uint32_t n) const
+    {
+        const uint64_t hi = (uint64_t(n) * m) >> 32;
+        return uint32_t((hi + n) >> s);
+    }
+};
+
+// Geometry and scalars every kernel needs (passed by value).
+struct KParams {
+    int nx, ny, nz;      // largest interior node index per axis
+    uint32_t N;          // interior nodes (nx+1)(ny+1)(nz+1)
+    FastDiv div_x;       // / (nx+1)
+    FastDiv div_y;       // / (ny+1)
+    int64_t o0;          // padded index of interior node (0,0,0)
+    int64_t Px, PxPy;    // padded strides of one y / z step
+    int64_t plane;       // values per population plane
+    int64_t rdy, rdz;    // reference-layout strides (InitUP perturbation index)
+    double ome, rho0, uu;
+};
+
+enum : int { K_COLLIDE = 1, K_WRITE_F = 2, K_WRITE_PU = 4, K_NOSHIFT = 8 };
+
+struct StepArgs {
+    KParams kp;
+    const void *src = nullptr;  // population input
+    void *dst = nullptr;        // post-collision output
+    void *fout = nullptr;       // array f output
+    double *p = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr;
+    const uint8_t *geo = nullptr;
+};
+
+// prec: 0 FP64, 1 FP32, 2 FP16 storage + FP32 arithmetic, 3 BF16 storage + FP32 arithmetic
+hipError_t launch_step(int dim, int prec, int mode, const StepArgs &a, hipStream_t s);
+hipError_t launch_collide(int dim, int prec, const StepArgs &a, hipStream_t s);
+hipError_t launch_init_pdf(int dim, int prec, const StepArgs &a, hipStream_t s);
+hipError_t launch_init_up(int dim, const StepArgs &a, const double *rv, hipStream_t s);
+hipError_t launch_fill(int prec, void *buf, const int64_t *dslot, const int64_t *sslot, int64_t n, hipStream_t s);
+hipError_t launch_monitor(int dim, const StepArgs &a, uint32_t skip, unsigned long long *out, hipStream_t s);
+
+}  // namespace imexlb

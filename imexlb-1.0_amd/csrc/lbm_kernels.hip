@@ -1,0 +1,499 @@
+// lbm_kernels.hip -- gfx950 kernels of the lattice Boltzmann time step.
+//
+// Device layout: populations are structure-of-arrays, one plane of `plane`
+// values per direction q, over a padded grid with ONE ghost layer:
+//     slot(q, x, y, z) = q*plane + o0 + x + Px*y + PxPy*z,  x in [-1, nx+1] ...
+// Interior fluid nodes hold post-collision values; interior solid nodes
+// hold 0 (the reference never writes them, lbm.f90:57); the ghost layer and
+// the body cells the reference rewrites (its b_user list) are filled once
+// per step from a resolved table (k_fill), which reproduces PassF plus the
+// five copy phases of BoundaryCondition (lbm.f90:96-155) slot for slot.
+//
+// One thread per interior node, x fastest.  In FP64 form every kernel keeps
+// the reference's operation order (lbm.f90:56-74, 163-199) and the library
+// is compiled without floating-point contraction, so results are the
+// reference's IEEE-754 results.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "kernels.hpp"
+
+namespace imexlb {
+
+// ---------------------------------------------------------------------------
+// lattice tables (reference numbering, initialization.f90:112-123 and
+// initialization_3d.f90:108-122)
+template <int D>
+struct Lat {
+    static constexpr int Q = D == 2 ? 9 : 27;
+    __host__ __device__ static constexpr int ex(int q) { return (q % 9) / 3 - 1; }
+    __host__ __device__ static constexpr int ey(int q) { return q % 3 - 1; }
+    __host__ __device__ static constexpr int ez(int q) { return D == 3 ? q / 9 - 1 : 0; }
+    __host__ __device__ static constexpr int norm1(int q)
+    {
+        return (ex(q) < 0 ? -ex(q) : ex(q)) + (ey(q) < 0 ? -ey(q) : ey(q)) + (ez(q) < 0 ? -ez(q) : ez(q));
+    }
+    // 2D: 4*(1-0.75ex^2)(1-0.75ey^2) is exact (4, 1, 0.25), then /9.
+    // 3D: 8/4^n/27.  Both divisions correctly rounded, as in the reference.
+    __host__ __device__ static constexpr double w(int q)
+    {
+        return D == 2 ? (norm1(q) == 0 ? 4.0 / 9.0 : norm1(q) == 1 ? 1.0 / 9.0 : 0.25 / 9.0)
+                      : (norm1(q) == 0 ? 8.0 / 27.0
+                                       : norm1(q) == 1 ? 2.0 / 27.0 : norm1(q) == 2 ? 0.5 / 27.0 : 0.125 / 27.0);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// storage forms of a population value
+struct bf16_t {
+    uint16_t bits;
+};
+
+template <class S>
+struct Store;
+template <>
+struct Store<double> {
+    __device__ static double ld(double v) { return v; }
+    template <class C>
+    __device__ static double st(C v) { return double(v); }
+};
+template <>
+struct Store<float> {
+    __device__ static float ld(float v) { return v; }
+    template <class C>
+    __device__ static float st(C v) { return float(v); }
+};
+template <>
+struct Store<_Float16> {
+    __device__ static float ld(_Float16 v) { return float(v); }
+    template <class C>
+    __device__ static _Float16 st(C v) { return _Float16(float(v)); }  // round to nearest even
+};
+template <>
+struct Store<bf16_t> {
+    __device__ static float ld(bf16_t v) { return __uint_as_float(uint32_t(v.bits) << 16); }
+    template <class C>
+    __device__ static bf16_t st(C v)
+    {
+        uint32_t u = __float_as_uint(float(v));
+        if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return bf16_t{uint16_t(0x7fc0)};
+        u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+        return bf16_t{uint16_t(u >> 16)};
+    }
+};
+
+// streaming (non-temporal) store of one population value
+template <class S>
+__device__ __forceinline__ void nt_store(S *p, S v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+template <>
+__device__ __forceinline__ void nt_store<bf16_t>(bf16_t *p, bf16_t v)
+{
+    __builtin_nontemporal_store(v.bits, &p->bits);
+}
+
+// ---------------------------------------------------------------------------
+// node decoding
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &d)
+{
+    return uint32_t((uint64_t(__umulhi(n, d.m)) + n) >> d.s);
+}
+
+// Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous
+// run of blocks so the cache lines two neighbouring blocks share stay in
+// one L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
+{
+    const uint32_t per = nb / 8u;
+    if (b >= per * 8u) return b;
+    return (b % 8u) * per + b / 8u;
+}
+
+template <int D>
+__device__ __forceinline__ bool decode(const KParams &kp, uint32_t i, int &x, int &y, int &z, uint32_t &c)
+{
+    if (i >= kp.N) return false;
+    const uint32_t r = fdiv(i, kp.div_x);
+    x = int(i - r * uint32_t(kp.nx + 1));
+    if (D == 3) {
+        const uint32_t zz = fdiv(r, kp.div_y);
+        y = int(r - zz * uint32_t(kp.ny + 1));
+        z = int(zz);
+    } else {
+        y = int(r);
+        z = 0;
+    }
+    c = uint32_t(kp.o0 + int64_t(x) + kp.Px * y + kp.PxPy * z);
+    return true;
+}
+
+template <int D>
+__device__ __forceinline__ int64_t shift(const KParams &kp, int q)
+{
+    using L = Lat<D>;
+    return int64_t(L::ex(q)) + kp.Px * L::ey(q) + kp.PxPy * L::ez(q);
+}
+
+// ---------------------------------------------------------------------------
+// PostProcessing arithmetic on one node (lbm.f90:163-199): moments summed in
+// q order, divided by 3 and by rho0, then the inlet Dirichlet velocity at
+// x = 0 and the outlet p = 0 at x = nx.
+template <int D, class C>
+__device__ __forceinline__ void moments(const C (&f)[Lat<D>::Q], const KParams &kp, int x, int y, int z, C &p,
+                                        C &ux, C &uy, C &uz)
+{
+    using L = Lat<D>;
+    C ps = C(0), sx = C(0), sy = C(0), sz = C(0);
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+        ps = ps + f[q];
+        if (L::ex(q) > 0) sx = sx + f[q];
+        if (L::ex(q) < 0) sx = sx - f[q];
+        if (L::ey(q) > 0) sy = sy + f[q];
+        if (L::ey(q) < 0) sy = sy - f[q];
+        if (D == 3 && L::ez(q) > 0) sz = sz + f[q];
+        if (D == 3 && L::ez(q) < 0) sz = sz - f[q];
+    }
+    const C rho0 = C(kp.rho0);
+    p = ps / C(3);
+    ux = sx / rho0;
+    uy = sy / rho0;
+    uz = D == 3 ? sz / rho0 : C(0);
+    if (x == 0) {  // inlet, lbm.f90:186-191 (3D: profile in z, u_z kept, lbm_3d.f90:188-193)
+        const int h = D == 2 ? y : z;
+        const int nh = D == 2 ? kp.ny : kp.nz;
+        ux = C(kp.uu) * rho0 * C(4) * C(h) * C(nh - h) / C(nh * nh);
+        uy = C(0);
+    }
+    if (x == kp.nx) p = C(0);  // outlet, lbm.f90:195-198
+}
+
+// Equilibrium (lbm.f90:70) and BGK relaxation (lbm.f90:71) of one node.
+// e.u is formed term by term in x, y, z order with the zero terms of the
+// reference's sum dropped (they only change the sign of a zero).
+template <int D, class C, class Emit>
+__device__ __forceinline__ void collide(const C (&f)[Lat<D>::Q], C p, C ux, C uy, C uz, const KParams &kp,
+                                        Emit emit)
+{
+    using L = Lat<D>;
+    const C ome = C(kp.ome), rho0 = C(kp.rho0);
+    const C omc = C(1) - ome;
+    C udu = ux * ux + uy * uy;
+    if (D == 3) udu = udu + uz * uz;
+    const C p3 = p * C(3);
+    const C u15 = C(1.5) * udu;
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+        C edu = C(0);
+        bool first = true;
+        auto term = [&](int e, C v) {
+            if (e == 0) return;
+            if (first) {
+                edu = e > 0 ? v : -v;
+                first = false;
+            } else {
+                edu = e > 0 ? edu + v : edu - v;
+            }
+        };
+        term(L::ex(q), ux);
+        term(L::ey(q), uy);
+        if (D == 3) term(L::ez(q), uz);
+        const C feq = C(L::w(q)) * (p3 + rho0 * ((C(3) * edu + C(4.5) * (edu * edu)) - u15));
+        emit(q, omc * f[q] + ome * feq);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The step kernel.  MODE bits:
+//   K_COLLIDE  relax and write dst (the next step's post-collision state)
+//   K_WRITE_F  write the gathered populations (array f)
+//   K_WRITE_PU write p and u (moments after the inlet/outlet conditions)
+//   K_NOSHIFT  read src at the node itself (PostProcessing of f in place)
+// COLLIDE alone is the fused Propagation+PostProcessing+Collision pull;
+// WRITE_F alone is Propagation; NOSHIFT|WRITE_PU is PostProcessing;
+// WRITE_F|WRITE_PU materialises f, p, u lazily.
+template <int D, class S, class C, int MODE>
+__global__ __launch_bounds__(256) void k_step(const S *__restrict__ src, S *__restrict__ dst,
+                                              S *__restrict__ fout, double *__restrict__ pout,
+                                              double *__restrict__ uxo, double *__restrict__ uyo,
+                                              double *__restrict__ uzo, const uint8_t *__restrict__ geo,
+                                              KParams kp)
+{
+    using L = Lat<D>;
+    int x, y, z;
+    uint32_t c;
+    if (!decode<D>(kp, xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, x, y, z, c)) return;
+    if (!geo[c]) {
+        if (MODE & K_WRITE_PU) {  // PostProcessing's inlet/outlet loops also reach solid nodes
+            if (x == 0) {
+                const int h = D == 2 ? y : z;
+                const int nh = D == 2 ? kp.ny : kp.nz;
+                uxo[c] = double(C(kp.uu) * C(kp.rho0) * C(4) * C(h) * C(nh - h) / C(nh * nh));
+                uyo[c] = 0.0;
+            }
+            if (x == kp.nx) pout[c] = 0.0;
+        }
+        return;
+    }
+    C f[L::Q];
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+        const S *sq = src + (int64_t(q) * kp.plane - ((MODE & K_NOSHIFT) ? 0 : shift<D>(kp, q)));
+        f[q] = Store<S>::ld(sq[c]);
+    }
+    if (MODE & K_WRITE_F) {
+#pragma unroll
+        for (int q = 0; q < L::Q; ++q) fout[int64_t(q) * kp.plane + c] = Store<S>::template st<C>(f[q]);
+    }
+    if (!(MODE & (K_WRITE_PU | K_COLLIDE))) return;
+    C p, ux, uy, uz;
+    moments<D, C>(f, kp, x, y, z, p, ux, uy, uz);
+    if (MODE & K_WRITE_PU) {
+        pout[c] = double(p);
+        uxo[c] = double(ux);
+        uyo[c] = double(uy);
+        if (D == 3) uzo[c] = double(uz);
+    }
+    if (MODE & K_COLLIDE) {
+        collide<D, C>(f, p, ux, uy, uz, kp, [&](int q, C v) {
+            nt_store(dst + int64_t(q) * kp.plane + c, Store<S>::template st<C>(v));
+        });
+    }
+}
+
+// Collision from stored f, p, u (lbm.f90:56-74): the first step after
+// InitPDF, and every step in per-phase mode.
+template <int D, class S, class C>
+__global__ __launch_bounds__(256) void k_collide(const S *__restrict__ fin, S *__restrict__ dst,
+                                                 const double *__restrict__ p, const double *__restrict__ ux,
+                                                 const double *__restrict__ uy, const double *__restrict__ uz,
+                                                 const uint8_t *__restrict__ geo, KParams kp)
+{
+    using L = Lat<D>;
+    int x, y, z;
+    uint32_t c;
+    if (!decode<D>(kp, xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, x, y, z, c)) return;
+    if (!geo[c]) return;
+    C f[L::Q];
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) f[q] = Store<S>::ld(fin[int64_t(q) * kp.plane + c]);
+    collide<D, C>(f, C(p[c]), C(ux[c]), C(uy[c]), D == 3 ? C(uz[c]) : C(0), kp,
+                  [&](int q, C v) { dst[int64_t(q) * kp.plane + c] = Store<S>::template st<C>(v); });
+}
+
+// InitPDF (lbm.f90:29-47): f = feq(p, u) on fluid nodes.
+template <int D, class S, class C>
+__global__ __launch_bounds__(256) void k_init_pdf(S *__restrict__ fout, const double *__restrict__ p,
+                                                  const double *__restrict__ ux, const double *__restrict__ uy,
+                                                  const double *__restrict__ uz, const uint8_t *__restrict__ geo,
+                                                  KParams kp)
+{
+    using L = Lat<D>;
+    int x, y, z;
+    uint32_t c;
+    if (!decode<D>(kp, xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, x, y, z, c)) return;
+    if (!geo[c]) return;
+    const C pp = C(p[c]), u0 = C(ux[c]), u1 = C(uy[c]), u2 = D == 3 ? C(uz[c]) : C(0);
+    C udu = u0 * u0 + u1 * u1;
+    if (D == 3) udu = udu + u2 * u2;
+    const C rho0 = C(kp.rho0);
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+        C edu = C(0);
+        bool first = true;
+        auto term = [&](int e, C v) {
+            if (e == 0) return;
+            if (first) {
+                edu = e > 0 ? v : -v;
+                first = false;
+            } else {
+                edu = e > 0 ? edu + v : edu - v;
+            }
+        };
+        term(L::ex(q), u0);
+        term(L::ey(q), u1);
+        if (D == 3) term(L::ez(q), u2);
+        const C feq = C(L::w(q)) * (pp * C(3) + rho0 * ((C(3) * edu + C(4.5) * (edu * edu)) - C(1.5) * udu));
+        fout[int64_t(q) * kp.plane + c] = Store<S>::template st<C>(feq);
+    }
+}
+
+// InitUP (initialization.f90:267-281; initialization_3d.f90:291-307).  The
+// perturbation index is the reference's own flat cell id modulo 100.
+template <int D>
+__global__ __launch_bounds__(256) void k_init_up(double *__restrict__ p, double *__restrict__ ux,
+                                                 double *__restrict__ uy, double *__restrict__ uz,
+                                                 const uint8_t *__restrict__ geo, const double *__restrict__ rv,
+                                                 KParams kp)
+{
+    int x, y, z;
+    uint32_t c;
+    if (!decode<D>(kp, blockIdx.x * blockDim.x + threadIdx.x, x, y, z, c)) return;
+    const int64_t rid = int64_t(x + 3) + kp.rdy * (y + 3) + (D == 3 ? kp.rdz * (z + 3) : 0);
+    const double r = 1.0 + (rv[rid % 100] - 0.5) * 2.0 * 0.2;
+    const double g = double(geo[c]);
+    p[c] = 0.0;
+    if (D == 2) {
+        const double yy = double(y);
+        ux[c] = kp.uu * 4.0 * yy * (double(kp.ny) - yy) / double(kp.ny * kp.ny) * g * r;
+        uy[c] = 0.0;
+    } else {
+        const double zz = double(z);
+        ux[c] = kp.uu * 4.0 * zz * (double(kp.nz) - zz) / double(kp.nz * kp.nz) * r * g;
+        uy[c] = 0.0;
+        uz[c] = 0.0;
+    }
+}
+
+// Boundary fill: dst slot <- resolved interior fluid slot (or 0).
+template <class S>
+__global__ __launch_bounds__(256) void k_fill(S *__restrict__ buf, const int64_t *__restrict__ dslot,
+                                              const int64_t *__restrict__ sslot, int64_t n)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t s = sslot[i];
+    buf[dslot[i]] = s >= 0 ? buf[s] : Store<S>::template st<float>(0.0f);
+}
+
+// Monitor (lbm.f90:212-217): max of ux^2+uy^2 over fluid nodes except the
+// first fluid node in scan order; sqrt is taken on the host (monotone, so
+// the maximum of the roots equals the root of the maximum).
+template <int D>
+__global__ __launch_bounds__(256) void k_monitor(const double *__restrict__ ux, const double *__restrict__ uy,
+                                                 const uint8_t *__restrict__ geo, uint32_t skip, KParams kp,
+                                                 unsigned long long *__restrict__ out)
+{
+    __shared__ double red[4];
+    int x, y, z;
+    uint32_t c;
+    double v = 0.0;
+    if (decode<D>(kp, blockIdx.x * blockDim.x + threadIdx.x, x, y, z, c) && geo[c] && c != skip)
+        v = ux[c] * ux[c] + uy[c] * uy[c];
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int w = 1; w < int(blockDim.x >> 6); ++w) m = fmax(m, red[w]);
+        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+
+// ---------------------------------------------------------------------------
+// host-side dispatch
+
+static inline unsigned nblocks(uint64_t n) { return unsigned((n + 255) / 256); }
+
+template <class F>
+static hipError_t dispatch_prec(int prec, F &&f)
+{
+    switch (prec) {
+    case 0: return f((double *)nullptr, (double *)nullptr);
+    case 1: return f((float *)nullptr, (float *)nullptr);
+    case 2: return f((_Float16 *)nullptr, (float *)nullptr);
+    case 3: return f((bf16_t *)nullptr, (float *)nullptr);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int D, int MODE>
+static hipError_t step_dm(int prec, const StepArgs &a, hipStream_t s)
+{
+    return dispatch_prec(prec, [&](auto *sp, auto *cp) {
+        using S = std::remove_pointer_t<decltype(sp)>;
+        using C = std::remove_pointer_t<decltype(cp)>;
+        hipLaunchKernelGGL((k_step<D, S, C, MODE>), dim3(nblocks(a.kp.N)), dim3(256), 0, s, (const S *)a.src,
+                           (S *)a.dst, (S *)a.fout, a.p, a.ux, a.uy, a.uz, a.geo, a.kp);
+        return hipGetLastError();
+    });
+}
+
+template <int D>
+static hipError_t step_d(int prec, int mode, const StepArgs &a, hipStream_t s)
+{
+    switch (mode) {
+    case K_COLLIDE: return step_dm<D, K_COLLIDE>(prec, a, s);
+    case K_WRITE_F: return step_dm<D, K_WRITE_F>(prec, a, s);
+    case K_WRITE_F | K_WRITE_PU: return step_dm<D, K_WRITE_F | K_WRITE_PU>(prec, a, s);
+    case K_NOSHIFT | K_WRITE_PU: return step_dm<D, K_NOSHIFT | K_WRITE_PU>(prec, a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_step(int dim, int prec, int mode, const StepArgs &a, hipStream_t s)
+{
+    return dim == 2 ? step_d<2>(prec, mode, a, s) : step_d<3>(prec, mode, a, s);
+}
+
+hipError_t launch_collide(int dim, int prec, const StepArgs &a, hipStream_t s)
+{
+    auto go = [&](auto dtag) {
+        constexpr int D = decltype(dtag)::value;
+        return dispatch_prec(prec, [&](auto *sp, auto *cp) {
+            using S = std::remove_pointer_t<decltype(sp)>;
+            using C = std::remove_pointer_t<decltype(cp)>;
+            hipLaunchKernelGGL((k_collide<D, S, C>), dim3(nblocks(a.kp.N)), dim3(256), 0, s, (const S *)a.src,
+                               (S *)a.dst, a.p, a.ux, a.uy, a.uz, a.geo, a.kp);
+            return hipGetLastError();
+        });
+    };
+    return dim == 2 ? go(std::integral_constant<int, 2>{}) : go(std::integral_constant<int, 3>{});
+}
+
+hipError_t launch_init_pdf(int dim, int prec, const StepArgs &a, hipStream_t s)
+{
+    auto go = [&](auto dtag) {
+        constexpr int D = decltype(dtag)::value;
+        return dispatch_prec(prec, [&](auto *sp, auto *cp) {
+            using S = std::remove_pointer_t<decltype(sp)>;
+            using C = std::remove_pointer_t<decltype(cp)>;
+            hipLaunchKernelGGL((k_init_pdf<D, S, C>), dim3(nblocks(a.kp.N)), dim3(256), 0, s, (S *)a.fout, a.p,
+                               a.ux, a.uy, a.uz, a.geo, a.kp);
+            return hipGetLastError();
+        });
+    };
+    return dim == 2 ? go(std::integral_constant<int, 2>{}) : go(std::integral_constant<int, 3>{});
+}
+
+hipError_t launch_init_up(int dim, const StepArgs &a, const double *rv, hipStream_t s)
+{
+    if (dim == 2)
+        hipLaunchKernelGGL((k_init_up<2>), dim3(nblocks(a.kp.N)), dim3(256), 0, s, a.p, a.ux, a.uy, a.uz, a.geo, rv,
+                           a.kp);
+    else
+        hipLaunchKernelGGL((k_init_up<3>), dim3(nblocks(a.kp.N)), dim3(256), 0, s, a.p, a.ux, a.uy, a.uz, a.geo, rv,
+                           a.kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(int prec, void *buf, const int64_t *dslot, const int64_t *sslot, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    return dispatch_prec(prec, [&](auto *sp, auto *) {
+        using S = std::remove_pointer_t<decltype(sp)>;
+        hipLaunchKernelGGL((k_fill<S>), dim3(nblocks(uint64_t(n))), dim3(256), 0, s, (S *)buf, dslot, sslot, n);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_monitor(int dim, const StepArgs &a, uint32_t skip, unsigned long long *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, out);
+    if (dim == 2)
+        hipLaunchKernelGGL((k_monitor<2>), dim3(nblocks(a.kp.N)), dim3(256), 0, s, a.ux, a.uy, a.geo, skip, a.kp,
+                           out);
+    else
+        hipLaunchKernelGGL((k_monitor<3>), dim3(nblocks(a.kp.N)), dim3(256), 0, s, a.ux, a.uy, a.geo, skip, a.kp,
+                           out);
+    return hipGetLastError();
+}
+
+}  // namespace imexlb

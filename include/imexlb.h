@@ -1,0 +1,168 @@
+/*
+ * imexlb.h -- C ABI of the MI355X lattice Boltzmann time step.
+ *
+ * The reference (IMEXLB-1.0) exposes this path as argument-less Fortran
+ * module procedures acting on module variables (lbm.f90, initialization.f90,
+ * cart_mpi.f90).  Each entry point below replaces one of them; the Fortran
+ * module shims in imexlb-1.0_amd/fortran bind these with ISO_C_BINDING so the
+ * reference's main.f90 call sequence drives the GPU unchanged.
+ *
+ * Conventions
+ *  - Every function returns IMEXLB_OK (0) or a negative IMEXLB_E_* code;
+ *    imexlb_strerror() describes it and imexlb_last_error() gives details.
+ *  - Inputs are copied; the caller keeps ownership of every pointer.
+ *  - A context owns all of its device memory and one HIP stream.  It is not
+ *    thread-safe.  Calls enqueue work asynchronously; imexlb_monitor and the
+ *    download functions synchronise.
+ *  - "Reference layout" (IMEXLB_LAYOUT_REF) is the reference's own host
+ *    layout: a flat array over array_size = prod(n_d+7) cells with a
+ *    3-wide ghost shell, cell id = (x+3) + dy*(y+3) [+ dz*(z+3)],
+ *    dy = nx+7, dz = (nx+7)*(ny+7); populations AoS f[id*nq+q]
+ *    (lbm.f90:44), velocity AoS u[id*dim+d] (lbm.f90:34).
+ *    "Interior layout" (IMEXLB_LAYOUT_INTERIOR) is structure-of-arrays over
+ *    the (nx+1)(ny+1)[(nz+1)] interior nodes, x fastest: f[q][z][y][x],
+ *    p[z][y][x], u[d][z][y][x].
+ *  - Lattice numbering is the reference's (initialization.f90:112-123;
+ *    initialization_3d.f90:108-122): D2Q9 q=(ex+1)*3+(ey+1),
+ *    D3Q27 q=(ez+1)*9+(ex+1)*3+(ey+1); the opposite of q is nq-1-q.
+ */
+#ifndef IMEXLB_H
+#define IMEXLB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct imexlb_ctx imexlb_ctx;
+
+/* status codes */
+#define IMEXLB_OK 0
+#define IMEXLB_E_ARG (-1)    /* invalid argument or size                      */
+#define IMEXLB_E_HIP (-2)    /* HIP runtime error                             */
+#define IMEXLB_E_OOM (-3)    /* device or host allocation failed              */
+#define IMEXLB_E_STATE (-4)  /* call out of the Collision..PostProcessing order */
+#define IMEXLB_E_UB (-5)     /* the reference's result depends on a read past
+                                the end of its arrays (undefined there)       */
+#define IMEXLB_E_NODEV (-6)  /* compute call on a host-only context (device<0) */
+
+/* storage / arithmetic forms */
+#define IMEXLB_F64 0       /* FP64 storage and arithmetic in the reference's
+                              operation order: results equal the reference's */
+#define IMEXLB_F32 1       /* FP32 storage, FP32 arithmetic                  */
+#define IMEXLB_F16_F32 2   /* FP16 storage of f, FP32 arithmetic              */
+#define IMEXLB_BF16_F32 3  /* BF16 storage of f, FP32 arithmetic              */
+
+#define IMEXLB_LAYOUT_REF 0
+#define IMEXLB_LAYOUT_INTERIOR 1
+
+#define IMEXLB_PDF_F 0   /* post-propagation populations  (array f)  */
+#define IMEXLB_PDF_FB 1  /* post-collision populations    (array fb) */
+
+/* list identifiers for imexlb_geometry_size / imexlb_geometry_list, in the
+ * reference's names (initialization.f90:44-50; initialization_3d.f90:44-50) */
+#define IMEXLB_LIST_FLUID 0
+#define IMEXLB_LIST_BL 1
+#define IMEXLB_LIST_BR 2
+#define IMEXLB_LIST_BU 3
+#define IMEXLB_LIST_BD 4
+#define IMEXLB_LIST_BF 5
+#define IMEXLB_LIST_BB 6
+#define IMEXLB_LIST_USER 7
+
+const char *imexlb_strerror(int code);
+/* text of the most recent error raised on this thread */
+const char *imexlb_last_error(void);
+
+/* omp_get_num_devices() as main.f90:31 uses it: number of visible GPUs. */
+int imexlb_device_count(int *n);
+
+/* MPISetup + InitLattice + AllocateArrays at one rank
+ * (cart_mpi.f90:47-138, initialization.f90:112-144; _3d counterparts).
+ * dim 2 (D2Q9, nz ignored) or 3 (D3Q27); n* are the largest node indices
+ * (nodes per axis = n+1, cart_mpi.f90:63-64).  device >= 0 selects the GPU
+ * (omp_set_default_device, main.f90:32); device = -1 makes a host-only
+ * context on which only the geometry functions work. */
+int imexlb_create(imexlb_ctx **ctx, int dim, int nx, int ny, int nz, int device, int precision);
+/* DeAllocateArrays + TARGET EXIT DATA (initialization.f90:148-163, main.f90:75) */
+int imexlb_destroy(imexlb_ctx *ctx);
+
+/* Scalars DataInput derives and the kernels use (initialization.f90:59-81):
+ * relaxation parameter ome, density rho0, inlet velocity scale uu. */
+int imexlb_set_params(imexlb_ctx *ctx, double ome, double rho0, double uu);
+
+/* SetGeometry (initialization.f90:167-255; initialization_3d.f90:170-279):
+ * cylinder/sphere mask centred at (pos, ny/2[, nz/2]), boundary lists,
+ * PassInt(geo), the body list b_user, and the device boundary tables. */
+int imexlb_set_geometry(imexlb_ctx *ctx, double pos, double radius);
+/* Sizes / contents of the lists SetGeometry builds, as 0-based reference
+ * cell ids (int32, like the reference's integer arrays). */
+int imexlb_geometry_size(imexlb_ctx *ctx, int list, int64_t *n);
+int imexlb_geometry_list(imexlb_ctx *ctx, int list, int32_t *out, int64_t cap);
+/* geo over array_size cells (reference layout, ghost shell included) */
+int imexlb_geometry_geo(imexlb_ctx *ctx, int32_t *geo, int64_t cap);
+/* Resolved boundary table: for every (cell, q) slot that a fluid node
+ * pulls from a ghost cell or a body cell, the reference slot id*nq+q and
+ * the interior fluid slot its value comes from after PassF and the five
+ * boundary phases (-1: the slot holds 0).  stats: [0] slots, [1] reads
+ * that depend on the order inside one boundary phase. */
+int imexlb_bc_table(imexlb_ctx *ctx, int64_t *slot, int64_t *src, int64_t cap, int64_t stats[2]);
+
+/* InitUP (initialization.f90:259-282; initialization_3d.f90:282-308) with
+ * the caller's 100 random numbers (the reference's r_vec(0:99)). */
+int imexlb_init_up(imexlb_ctx *ctx, const double *r_vec);
+/* Warm start: replace p and u (layout IMEXLB_LAYOUT_*), e.g. with a
+ * predicted field; follow with imexlb_init_pdf (lbm.f90:24-48). */
+int imexlb_upload_fields(imexlb_ctx *ctx, const double *p, const double *u, int layout);
+/* InitPDF (lbm.f90:24-48): f = feq(p, u) on fluid nodes. */
+int imexlb_init_pdf(imexlb_ctx *ctx);
+
+/* The four phases of one time step, main.f90:57-61. */
+int imexlb_collision(imexlb_ctx *ctx);           /* Collision          lbm.f90:52-76   */
+int imexlb_boundary_condition(imexlb_ctx *ctx);  /* BoundaryCondition  lbm.f90:96-155,
+                                                    incl. PassF        cart_mpi.f90:240 */
+int imexlb_propagation(imexlb_ctx *ctx);         /* Propagation        lbm.f90:80-92   */
+int imexlb_post_processing(imexlb_ctx *ctx);     /* PostProcessing     lbm.f90:159-201 */
+
+/* Choose how the four phases execute: fused = 1 (default) runs
+ * Propagation+PostProcessing+next Collision as one pull kernel and
+ * materialises f/p/u only when read; fused = 0 runs one kernel per phase.
+ * Both give identical results. */
+int imexlb_set_fused(imexlb_ctx *ctx, int fused);
+
+/* nsteps complete iterations (Collision, BoundaryCondition, Propagation,
+ * PostProcessing), i.e. nsteps trips of the loop body at main.f90:48-63
+ * without the Monitor call. */
+int imexlb_advance(imexlb_ctx *ctx, int64_t nsteps);
+
+/* Monitor's local maximum (lbm.f90:205-217): max over fluid entries 1..F-1
+ * of sqrt(ux^2+uy^2).  Synchronises. */
+int imexlb_monitor(imexlb_ctx *ctx, double *um);
+
+/* TARGET UPDATE from(u,p) (lbm.f90:244): p and u of the current state. */
+int imexlb_download_fields(imexlb_ctx *ctx, double *p, double *u, int layout);
+/* f (IMEXLB_PDF_F) or fb (IMEXLB_PDF_FB) of the current state, as FP64.
+ * Reference layout writes interior cells only; fb ghost values are the
+ * resolved ones where a fluid node reads them. */
+int imexlb_download_pdf(imexlb_ctx *ctx, int which, double *out, int layout);
+
+/* Wait for all work queued on the context's stream. */
+int imexlb_synchronize(imexlb_ctx *ctx);
+
+/* Kernel timing with HIP events on the context stream.  While enabled,
+ * every launch is bracketed by events; imexlb_profile_read returns, for
+ * kernel class k (0 fused step, 1 boundary fill, 2 other), the launch count
+ * and the summed milliseconds since the last reset. */
+int imexlb_profile(imexlb_ctx *ctx, int enable);
+int imexlb_profile_read(imexlb_ctx *ctx, int64_t launches[3], double ms[3]);
+
+/* Sizes: [0] interior nodes, [1] fluid nodes, [2] bytes of one population
+ * array on the device, [3] device bytes allocated, [4] storage bytes per
+ * population value, [5] boundary-table entries. */
+int imexlb_info(imexlb_ctx *ctx, int64_t out[6]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IMEXLB_H */

@@ -1,0 +1,143 @@
+"""Benchmark: lattice updates per second of the MI355X time step.
+
+Workload (one GPU): the D3Q27 sphere, 512x256x256 (513x257x257 = 33,883,137
+nodes), Re 200, in the reference's own boundary semantics, FP64 storage and
+arithmetic (results equal to the reference's).  A step is one trip of the
+reference's loop body (Collision, BoundaryCondition, Propagation,
+PostProcessing) over the whole lattice; inputs are resident in HBM before
+the timed region.  The CPU baseline is the oracle (the C restatement of the
+reference) run on the host cores over a few steps of the same lattice.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-variants]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "imexlb-1.0_amd"))
+
+import numpy as np  # noqa: E402
+
+import imexlb as I  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+DECK = "sphere512x256x256"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def gpu_leg(deck, r_vec, precision, steps, warmup):
+    lat, d = I.setup(deck, r_vec, device=0, precision=precision)
+    info = lat.info()
+    lat.advance(warmup)
+    lat.synchronize()
+    lat.profile(True)
+    t0 = time.perf_counter()
+    lat.advance(steps)
+    lat.synchronize()
+    t1 = time.perf_counter()
+    n, ms = lat.profile_read()
+    lat.profile(False)
+    um = lat.monitor()
+    lat.close()
+    return {"seconds": t1 - t0, "fused_launches": n[0], "fused_ms": ms[0], "fill_launches": n[1],
+            "fill_ms": ms[1], "other_ms": ms[2], "info": info, "umax": um}
+
+
+def cpu_leg(deck, r_vec, steps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    d = I.derive(deck)
+    o = O.Oracle(deck.dim, *deck.n)
+    o.set_params(deck.ome, deck.rho0, d.uu)
+    o.set_geometry(d.pos, d.radius)
+    o.init_up(r_vec)
+    o.init_pdf()
+    t0 = time.perf_counter()
+    o.run(0, steps - 1, 0)
+    t1 = time.perf_counter()
+    o.close()
+    return t1 - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the lattice is not decomposed across GPUs).", file=sys.stderr)
+        sys.exit(2)
+
+    deck = I.deck(DECK)
+    nodes = I.nodes(deck)
+    Q = 27
+    r_vec = np.random.default_rng(2021).random(100)
+
+    log(f"[bench] {DECK} FP64: setup, {args.warmup} warm-up + {args.steps} timed steps")
+    g = gpu_leg(deck, r_vec, I.F64, args.steps, args.warmup)
+    sec = g["seconds"]
+    mlups = nodes * args.steps / sec / 1e6
+    avg_ms = g["fused_ms"] / max(g["fused_launches"], 1)
+    bytes_per_launch = nodes * 2 * Q * 8
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    log(f"[bench] FP64 {mlups:.0f} MLUPS, fused kernel {avg_ms:.3f} ms -> {achieved:.0f} GB/s")
+
+    variants = {}
+    if not args.no_variants:
+        for name, prec, s in (("f32", I.F32, 4), ("f16_f32", I.F16_F32, 2), ("bf16_f32", I.BF16_F32, 2)):
+            log(f"[bench] {DECK} {name}: {args.steps} timed steps")
+            v = gpu_leg(deck, r_vec, prec, args.steps, args.warmup)
+            vm = nodes * args.steps / v["seconds"] / 1e6
+            va = v["fused_ms"] / max(v["fused_launches"], 1)
+            variants[name] = {"MLUPS": round(vm, 1), "fused_kernel_ms": round(va, 4),
+                              "achieved_GBps": round(nodes * 2 * Q * s / (va * 1e-3) / 1e9, 1),
+                              "speedup_vs_f64": round(vm / mlups, 3)}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        log(f"[bench] CPU oracle: {args.cpu_steps} steps of the same lattice on {threads} threads")
+        cs = cpu_leg(deck, r_vec, args.cpu_steps)
+        cpu = {"value": round(nodes * args.cpu_steps / cs / 1e6, 2), "unit": "MLUPS", "cores": threads,
+               "kind": "port",
+               "sample": f"{args.cpu_steps} full steps of the 513x257x257 sphere lattice (oracle, OpenMP)"}
+
+    out = {
+        "metric": "MLUPS (D3Q27 sphere, FP64) & % HBM roofline",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: InitUP perturbation from a seeded generator; geometry and boundaries as the reference",
+        "config": {"workload": "D3Q27 BGK sphere 512x256x256 (513x257x257 nodes), radius 20, Re 200, FP64",
+                   "nodes": nodes, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "kernel": {"name": "k_step<3,double,double,1>", "avg_ms": round(avg_ms, 4),
+                   "launches": g["fused_launches"], "algorithmic_bytes_per_launch": bytes_per_launch,
+                   "fill_avg_ms": round(g["fill_ms"] / max(g["fill_launches"], 1), 4)},
+        "variants": variants,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -436,6 +436,10 @@ int imexlb_set_geometry(imexlb_ctx *c, double pos, double radius)
             src.push_back(int64_t(sq) * c->plane + c->pidx(x, y, z));
         }
     }
+    const int64_t nslots = int64_t(c->Q) * c->plane;
+    for (size_t i = 0; i < dst.size(); ++i)
+        if (dst[i] < 0 || dst[i] >= nslots || src[i] < -1 || src[i] >= nslots)
+            return fail(IMEXLB_E_ARG, "boundary table slot outside the population arrays");
     if (c->fill_dst) (void)hipFree(c->fill_dst);
     if (c->fill_src) (void)hipFree(c->fill_src);
     c->fill_dst = c->fill_src = nullptr;

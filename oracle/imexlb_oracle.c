@@ -472,7 +472,7 @@ void orc_propagation(orc *o)
  * left as computed), then outlet p(nx,.) = 0.
  * Pinned by: closed-form moments of hand-built populations.
  * ---------------------------------------------------------------------- */
-void orc_post_processing(orc *o)
+void orc_moments(orc *o)
 {
     const int dim = o->dim, nq = o->nq;
 #pragma omp parallel for schedule(static)
@@ -487,6 +487,12 @@ void orc_post_processing(orc *o)
         for (int d = 0; d < dim; ++d) o->u[id * dim + d] = u[d] / o->rho0 * (double)o->geo[id];
         o->p[id] = p / 3.0 * (double)o->geo[id];
     }
+}
+
+/* the inlet and outlet conditions PostProcessing applies after the moments */
+void orc_post_bc(orc *o)
+{
+    const int dim = o->dim;
     const int ny = o->n[1], nz = o->n[2];
     for (int64_t idn = 0; idn < o->l_size; ++idn) {
         int64_t id = o->bl[idn] + 1;
@@ -501,6 +507,12 @@ void orc_post_processing(orc *o)
         }
     }
     for (int64_t idn = 0; idn < o->r_size; ++idn) o->p[o->br[idn] - 1] = 0.0;
+}
+
+void orc_post_processing(orc *o)
+{
+    orc_moments(o);
+    orc_post_bc(o);
 }
 
 /* ------------------------------------------------------------------------

@@ -38,7 +38,7 @@ def lib():
         L.orc_set_geometry.argtypes = [P, C.c_double, C.c_double]
         L.orc_init_up.argtypes = [P, C.POINTER(C.c_double)]
         for name in ("orc_init_pdf", "orc_collision", "orc_boundary_condition",
-                     "orc_propagation", "orc_post_processing"):
+                     "orc_propagation", "orc_post_processing", "orc_moments", "orc_post_bc"):
             getattr(L, name).argtypes = [P]
         L.orc_pass_f.argtypes = [P, P]
         L.orc_pass_int.argtypes = [P, P]
@@ -201,6 +201,10 @@ class Oracle:
 
     def post_processing(self):
         lib().orc_post_processing(self._h)
+
+    def moments(self):
+        """PostProcessing's moment pass alone (no inlet/outlet conditions)."""
+        lib().orc_moments(self._h)
 
     def monitor(self):
         return lib().orc_monitor(self._h)

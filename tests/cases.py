@@ -1,0 +1,67 @@
+"""Shared test decks and helpers (4-record input format of input.in)."""
+import numpy as np
+
+import imexlb as I
+import oracle as O
+
+# small decks that the oracle runs in well under a second per step
+TINY = {
+    # cylinder of radius 6 at (12, 12) in a 49x25 channel
+    "tiny2d": (2, "48 24\n0.25 0.25 20\n1.6 1 1\n0.02 0.001\n"),
+    # sphere of radius 3 at (6, 6, 6) in a 25x13x13 box
+    "tiny3d": (3, "24 12 12\n0.25 0.25 20\n1.6 1 1\n0.05 0.005\n"),
+    # body cut by the inlet plane and both walls: disc of radius 10 centred
+    # on the inlet column x = 0 of a 41x21 channel
+    "inletbody2d": (2, "40 20\n0.5 0.0 20\n1.5 1 1\n0.02 0.001\n"),
+    # sphere touching the z walls: radius 6 centred at (8, 6, 6) in 25x13x13
+    "wallsphere3d": (3, "24 12 12\n0.5 0.333333333333333 20\n1.5 1 1\n0.02 0.001\n"),
+}
+
+
+def get_deck(name):
+    if name in TINY:
+        dim, text = TINY[name]
+        return I.parse_deck(text, dim)
+    return I.deck(name)
+
+
+def oracle_setup(deck, r_vec):
+    d = I.derive(deck)
+    o = O.Oracle(deck.dim, *deck.n)
+    o.set_params(deck.ome, deck.rho0, d.uu)
+    o.set_geometry(d.pos, d.radius)
+    o.init_up(r_vec)
+    o.init_pdf()
+    return o, d
+
+
+def interior_mask(o):
+    ids = np.arange(o.A)
+    nx, ny, nz = o.n
+    x = ids % o.dyy - 3
+    y = (ids // o.dyy) % (ny + 7) - 3
+    m = (x >= 0) & (x <= nx) & (y >= 0) & (y <= ny)
+    if o.dim == 3:
+        z = ids // o.dzz - 3
+        m &= (z >= 0) & (z <= nz)
+    return m
+
+
+def assert_state_equal(lat, o, what=("f", "p", "u")):
+    """Bit-for-bit equality (as IEEE values; +0 == -0) of f, p, u on interior cells."""
+    m = interior_mask(o)
+    if "f" in what:
+        f_dev = lat.download_pdf(I.PDF_F, I.LAYOUT_REF).reshape(o.A, o.nq)
+        f_or = o.f.reshape(o.A, o.nq)
+        bad = np.nonzero(~(f_dev[m] == f_or[m]).all(axis=1))[0]
+        assert bad.size == 0, f"f differs on {bad.size} interior cells, first ids {np.nonzero(m)[0][bad[:5]]}"
+    if "p" in what or "u" in what:
+        p_dev, u_dev = lat.download_fields(I.LAYOUT_REF)
+        if "p" in what:
+            bad = np.nonzero(p_dev[m] != o.p[m])[0]
+            assert bad.size == 0, f"p differs on {bad.size} cells, first ids {np.nonzero(m)[0][bad[:5]]}"
+        if "u" in what:
+            ud = u_dev.reshape(o.A, o.dim)[m]
+            uo = o.u.reshape(o.A, o.dim)[m]
+            bad = np.nonzero(~(ud == uo).all(axis=1))[0]
+            assert bad.size == 0, f"u differs on {bad.size} cells, first ids {np.nonzero(m)[0][bad[:5]]}"

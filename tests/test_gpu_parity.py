@@ -1,0 +1,144 @@
+"""Parity of the HIP path (through the C ABI) with the oracle.
+
+FP64 form: bit-for-bit equality of f, p, u and of every Monitor value after
+N steps (IEEE values; the only admitted difference is the sign of a zero).
+Low-precision forms: error against the FP64 HIP run, bounded as stated per
+test.
+"""
+import numpy as np
+import pytest
+
+import imexlb as I
+from cases import assert_state_equal, get_deck, oracle_setup
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_pair(name, r_vec, steps, fused=True, precision=I.F64):
+    deck = get_deck(name)
+    print(f"[parity] deck={name} steps={steps} fused={fused} precision={precision}", flush=True)
+    lat, d = I.setup(deck, r_vec, device=0, precision=precision)
+    lat.set_fused(fused)
+    o, _ = oracle_setup(deck, r_vec)
+    return lat, o, d
+
+
+def _advance_with_monitor(lat, o, d, steps):
+    """main.f90:48-63 for iter = 0..steps-1 on both sides; returns Monitor series."""
+    interv = max(d.interv, 1)
+    um_dev, um_or = [], []
+    it = 0
+    while it < steps:
+        if it % interv == 0:
+            um_dev.append(lat.monitor())
+        n = min(interv - it % interv, steps - it)
+        lat.advance(n)
+        it += n
+    um_or = list(o.run(0, steps - 1, interv))
+    return np.array(um_dev), np.array(um_or)
+
+
+@pytest.mark.parametrize("name,steps", [("tiny2d", 70), ("tiny3d", 44), ("inletbody2d", 145),
+                                        ("wallsphere3d", 52)])
+@pytest.mark.parametrize("fused", [True, False])
+def test_full_run_bitwise_small(name, steps, fused, r_vec):
+    lat, o, d = _run_pair(name, r_vec, steps, fused)
+    um_dev, um_or = _advance_with_monitor(lat, o, d, steps)
+    assert np.array_equal(um_dev, um_or), (um_dev, um_or)
+    assert_state_equal(lat, o)
+    lat.close()
+    o.close()
+
+
+def test_cylinder_400x100_bitwise(r_vec):
+    """The 400x100 cylinder at Re 100: 1000 steps, Monitor every 100."""
+    lat, o, d = _run_pair("cyl400x100", r_vec, 1000)
+    um_dev, um_or = _advance_with_monitor(lat, o, d, 1000)
+    assert len(um_dev) == 10
+    assert np.array_equal(um_dev, um_or), (um_dev, um_or)
+    assert_state_equal(lat, o)
+    lat.close()
+    o.close()
+
+
+def test_reference_inputs_bitwise(r_vec):
+    """The reference's own example decks input.in (200 steps) and input_3d.in (6 steps)."""
+    for name, steps in (("input.in", 200), ("input_3d.in", 6)):
+        lat, o, d = _run_pair(name, r_vec, steps)
+        lat.advance(steps)
+        o.run(0, steps - 1, 0)
+        assert lat.monitor() == o.monitor()
+        assert_state_equal(lat, o)
+        lat.close()
+        o.close()
+
+
+def test_phase_by_phase(r_vec):
+    """State after each of Collision, BoundaryCondition, Propagation and
+    PostProcessing (unfused), and after the fused step, equals the oracle's."""
+    deck = get_deck("tiny2d")
+    lat, d = I.setup(deck, r_vec, device=0)
+    lat.set_fused(False)
+    o, _ = oracle_setup(deck, r_vec)
+    fid = o.list("fluid_id")
+    slot, src, _ = lat.bc_table()
+    for step in range(3):
+        lat.collision(); o.collision()
+        fb = lat.download_pdf(I.PDF_FB, I.LAYOUT_REF).reshape(o.A, o.nq)
+        assert np.array_equal(fb[fid], o.fb.reshape(o.A, o.nq)[fid])
+        lat.boundary_condition(); o.boundary_condition()
+        fbf = lat.download_pdf(I.PDF_FB, I.LAYOUT_REF)
+        assert np.array_equal(fbf[slot], o.fb[slot])
+        lat.propagation(); o.propagation()
+        assert_state_equal(lat, o, what=("f",))
+        lat.post_processing(); o.post_processing()
+        assert_state_equal(lat, o)
+    lat.close()
+    o.close()
+
+
+def test_fused_and_unfused_agree_midstep(r_vec):
+    """Downloads between phases in fused mode see the reference's state."""
+    deck = get_deck("tiny3d")
+    lat, d = I.setup(deck, r_vec, device=0)
+    o, _ = oracle_setup(deck, r_vec)
+    for _ in range(2):
+        lat.advance(1)
+        o.run(0, 0, 0)
+    lat.collision(); o.collision()
+    assert_state_equal(lat, o)  # f, p, u of this step are still readable after Collision
+    lat.boundary_condition(); o.boundary_condition()
+    lat.propagation(); o.propagation()
+    lat.post_processing(); o.post_processing()
+    assert_state_equal(lat, o)
+    lat.close()
+    o.close()
+
+
+def test_state_errors(r_vec):
+    deck = get_deck("tiny2d")
+    lat, d = I.setup(deck, r_vec, device=0)
+    with pytest.raises(I.ImexlbError):
+        lat.propagation()  # before Collision and BoundaryCondition
+    lat.collision()
+    with pytest.raises(I.ImexlbError):
+        lat.post_processing()
+    lat.close()
+
+
+@pytest.mark.parametrize("precision,tol", [(I.F32, 1e-5), (I.F16_F32, 3e-2), (I.BF16_F32, 2e-1)])
+def test_precision_forms_error(precision, tol, r_vec):
+    """FP32 arithmetic with FP32 / FP16 / BF16 storage against the FP64 run of
+    the 400x100 cylinder after 2000 steps: relative L2 error of u below tol."""
+    deck = get_deck("cyl400x100")
+    ref, _ = I.setup(deck, r_vec, device=0)
+    ref.advance(2000)
+    _, u64 = ref.download_fields()
+    lat, _ = I.setup(deck, r_vec, device=0, precision=precision)
+    lat.advance(2000)
+    _, u = lat.download_fields()
+    err = np.linalg.norm(u - u64) / np.linalg.norm(u64)
+    print(f"precision {precision}: rel L2 error of u = {err:.3e}")
+    assert np.isfinite(err) and err < tol
+    ref.close()
+    lat.close()

@@ -1,0 +1,141 @@
+"""Host-side logic of the product, on the CPU (no compute calls without a GPU):
+
+* libimexlb.so loads and exports every entry point include/imexlb.h declares;
+* input decks parse like the reference's list-directed read;
+* a host-only context (device -1) builds the reference's mask and lists
+  exactly like the oracle's SetGeometry;
+* the resolved boundary table names, for every slot a fluid node pulls from a
+  ghost or body cell, the value the oracle's literal PassF + five copy phases
+  leave there;
+* errors are reported as status codes.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import imexlb as I
+from imexlb import _lib
+from cases import get_deck, interior_mask
+import oracle as O
+
+LISTS = ("fluid_id", "bl", "br", "bu", "bd", "bf", "bb", "b_user")
+
+
+def test_library_exports_every_declared_symbol(root):
+    header = open(os.path.join(root, "include", "imexlb.h")).read()
+    declared = set(re.findall(r"^\s*(?:int|const char \*)\s*(imexlb_\w+)\s*\(", header, re.M))
+    assert len(declared) >= 25
+    L = C.CDLL(_lib.LIB_PATH)
+    for name in sorted(declared):
+        assert hasattr(L, name), f"{name} declared but not exported"
+    assert declared == set(_lib.SIGNATURES), "ctypes signatures out of step with the header"
+
+
+def test_parse_deck_list_directed():
+    text = "512\t128\n0.03125\t0.125\t100\n1.953125\t1\t1\n0.5\t0.01\n\n----\nnx ny\nL Xpos Re\n"
+    d = I.parse_deck(text, 2)
+    assert d.n == (512, 128, 0) and d.Re == 100.0 and d.max_t == 0.5 and d.inter == 0.01
+    d3 = I.parse_deck("512 128\n128\n0.03125 0.125 300 extra\n1.953125,1,1\n0.1 0.01\n", 3)
+    assert d3.n == (512, 128, 128) and d3.Re == 300.0 and d3.rho0 == 1.0
+    assert I.parse_deck("8 8\n1.d0 0.5 10\n1.2 1 1\n1 0.1\n", 2).Lc == 1.0
+    with pytest.raises(ValueError):
+        I.parse_deck("8 8\n1 0.5 10\n", 2)
+
+
+def _host_ctx(deck):
+    d = I.derive(deck)
+    lat = I.Lattice(deck.dim, *deck.n, device=-1)
+    lat.set_params(deck.ome, deck.rho0, d.uu)
+    lat.set_geometry(d.pos, d.radius)
+    o = O.Oracle(deck.dim, *deck.n)
+    o.set_params(deck.ome, deck.rho0, d.uu)
+    o.set_geometry(d.pos, d.radius)
+    return lat, o
+
+
+@pytest.mark.parametrize("name", ["tiny2d", "tiny3d", "inletbody2d", "wallsphere3d", "cyl400x100", "input.in"])
+def test_geometry_matches_oracle(name):
+    lat, o = _host_ctx(get_deck(name))
+    assert np.array_equal(lat.geometry_geo(), o.geo)
+    for k in LISTS:
+        assert np.array_equal(lat.geometry_list(k), o.list(k)), k
+    lat.close()
+    o.close()
+
+
+def _check_bc_table(lat, o):
+    slot, src, stats = lat.bc_table()
+    fb = o.fb.reshape(o.A, o.nq)
+    fb[:] = -7.0e9
+    m = interior_mask(o)
+    fb[m & (o.geo == 0)] = 0.0
+    fid = o.list("fluid_id")
+    fb[fid] = (fid[:, None] * o.nq + np.arange(o.nq)[None, :] + 1).astype(float)
+    o.boundary_condition()
+    expect = np.where(src >= 0, (src + 1).astype(float), 0.0)
+    assert np.array_equal(o.fb[slot], expect)
+    # completeness: exactly the (fluid node, q) pulls from a ghost or body cell
+    is_user = np.zeros(o.A, dtype=bool)
+    is_user[o.list("b_user")] = True
+    want = []
+    for q in range(o.nq):
+        off = o.e[q, 0] + o.dyy * o.e[q, 1] + (o.dzz * o.e[q, 2] if o.dim == 3 else 0)
+        s = fid - off
+        sel = ~m[s] | is_user[s]
+        want.append(s[sel] * o.nq + q)
+    assert np.array_equal(np.sort(np.concatenate(want)), np.sort(slot))
+    return stats
+
+
+@pytest.mark.parametrize("name", ["tiny2d", "tiny3d", "inletbody2d", "wallsphere3d", "cyl400x100", "input.in"])
+def test_bc_table_matches_oracle_boundary_condition(name):
+    lat, o = _host_ctx(get_deck(name))
+    stats = _check_bc_table(lat, o)
+    if name != "wallsphere3d":  # that body touches the walls: order-dependent reads exist there
+        assert stats["order_sensitive"] == 0
+    lat.close()
+    o.close()
+
+
+@pytest.mark.slow
+def test_bc_table_reference_3d_input():
+    lat, o = _host_ctx(I.deck("input_3d.in"))
+    stats = _check_bc_table(lat, o)
+    assert stats["order_sensitive"] == 0 and stats["slots"] == 2674879
+    lat.close()
+    o.close()
+
+
+def test_error_codes():
+    with pytest.raises(I.ImexlbError) as e:
+        I.Lattice(4, 10, 10, 10, device=-1)
+    assert e.value.code == _lib.E_ARG
+    with pytest.raises(I.ImexlbError) as e:
+        I.Lattice(2, 1, 10, device=-1)  # fewer than 3 nodes (cart_mpi.f90:107-110)
+    assert e.value.code == _lib.E_ARG
+    lat = I.Lattice(2, 10, 10, device=-1)
+    with pytest.raises(I.ImexlbError) as e:
+        lat.geometry_list("fluid_id")  # before SetGeometry
+    assert e.value.code == _lib.E_STATE
+    lat.set_params(1.5, 1.0, 0.01)
+    lat.set_geometry(4.0, 2.0)
+    for call in (lat.collision, lat.init_pdf, lat.monitor, lambda: lat.advance(1)):
+        with pytest.raises(I.ImexlbError) as e:
+            call()
+        assert e.value.code == _lib.E_NODEV
+    with pytest.raises(I.ImexlbError):
+        lat.set_params(1.5, 0.0, 0.01)
+    lat.close()
+
+
+def test_info_sizes():
+    lat = I.Lattice(3, 24, 12, 12, device=-1)
+    lat.set_params(1.5, 1.0, 0.01)
+    lat.set_geometry(6.0, 3.0)
+    info = lat.info()
+    assert info["nodes"] == 25 * 13 * 13 and info["value_bytes"] == 8
+    assert info["fluid"] == len(lat.geometry_list("fluid_id"))
+    lat.close()

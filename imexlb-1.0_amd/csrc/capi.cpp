@@ -132,9 +132,12 @@ struct imexlb_ctx {
         return end(cls, b);
     }
 
+    // BoundaryCondition on buffer `buf`; the other buffer holds the previous
+    // step's state (every Collision writes the buffer the previous step did
+    // not use)
     int run_fill(int buf)
     {
-        return launch(1, [&] { return launch_fill(prec, fb[buf], fill_dst, fill_src, nfill, stream); });
+        return launch(1, [&] { return launch_fill(prec, fb[buf], fb[1 - buf], fill_dst, fill_src, nfill, stream); });
     }
 
     int run_step(int mode, const void *src, void *dst, void *fout)
@@ -421,25 +424,23 @@ int imexlb_set_geometry(imexlb_ctx *c, double pos, double radius)
     std::vector<int64_t> dst, src;
     dst.reserve(c->bc.entries.size());
     src.reserve(c->bc.entries.size());
-    for (const FillEntry &e : c->bc.entries) {
+    auto dev_slot = [&](int64_t ref_slot) {
         int x, y, z;
-        const int64_t cell = e.ref_slot / c->Q;
-        const int q = int(e.ref_slot % c->Q);
-        g.coords(cell, x, y, z);
-        dst.push_back(int64_t(q) * c->plane + c->pidx(x, y, z));
-        if (e.ref_src < 0) {
-            src.push_back(-1);
-        } else {
-            const int64_t sc = e.ref_src / c->Q;
-            const int sq = int(e.ref_src % c->Q);
-            g.coords(sc, x, y, z);
-            src.push_back(int64_t(sq) * c->plane + c->pidx(x, y, z));
-        }
-    }
+        g.coords(ref_slot / c->Q, x, y, z);
+        return int64_t(ref_slot % c->Q) * c->plane + c->pidx(x, y, z);
+    };
     const int64_t nslots = int64_t(c->Q) * c->plane;
-    for (size_t i = 0; i < dst.size(); ++i)
-        if (dst[i] < 0 || dst[i] >= nslots || src[i] < -1 || src[i] >= nslots)
+    for (const FillEntry &e : c->bc.entries) {
+        const int64_t d = dev_slot(e.ref_slot);
+        int64_t s = -1;  // zero
+        if (e.ref_src >= 0) s = dev_slot(e.ref_src);
+        else if (e.ref_src <= -2) s = encode_prev(dev_slot(decode_prev(e.ref_src)));
+        const int64_t sabs = s >= 0 ? s : s == -1 ? 0 : decode_prev(s);
+        if (d < 0 || d >= nslots || sabs < 0 || sabs >= nslots)
             return fail(IMEXLB_E_ARG, "boundary table slot outside the population arrays");
+        dst.push_back(d);
+        src.push_back(s);
+    }
     if (c->fill_dst) (void)hipFree(c->fill_dst);
     if (c->fill_src) (void)hipFree(c->fill_src);
     c->fill_dst = c->fill_src = nullptr;
@@ -589,11 +590,14 @@ int imexlb_collision(imexlb_ctx *c)
         return IMEXLB_OK;
     }
     if (c->stage != S_PRE) return fail(IMEXLB_E_STATE, "Collision needs InitPDF or a completed step");
+    // write the buffer the previous step did not use: its body cells are
+    // the previous values BoundaryCondition may read
     StepArgs a = c->args();
     a.src = c->f;
-    a.dst = c->fb[c->cur];
+    a.dst = c->fb[1 - c->cur];
     int rc = c->launch(2, [&] { return launch_collide(c->dim, c->prec, a, c->stream); });
     if (rc) return rc;
+    c->cur = 1 - c->cur;
     c->have_fb = true;
     c->stage = S_COLLIDED;
     return IMEXLB_OK;

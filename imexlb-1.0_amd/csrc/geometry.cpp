@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <unordered_map>
 
 namespace imexlb {
 
@@ -175,14 +176,20 @@ struct Resolver {
         return s >= 0 && s < g.A;
     }
 
-    // Value of slot (c,q) after phases 0..k: >= 0 the reference slot of the
-    // interior fluid value it equals, -1 zero, -2 undefined.
-    int64_t value(int64_t c, int q, int k)
+    // Value of slot (c,q) after phases 0..k, in FillEntry::ref_src encoding.
+    // Before PassF (k < 0) an interior cell holds its post-collision value if
+    // fluid, the previous step's value if a body cell, 0 otherwise.
+    int64_t value(int64_t c, int q, int k, bool &undefined)
     {
         for (;;) {
             if (k < 0) {
-                if (ghost(c)) return -2;  // stale ghost value: never reached in practice
-                return g.geo[c] == 1 ? c * g.nq + q : -1;
+                if (ghost(c)) {  // a stale ghost value: PassF refreshes every ghost cell first
+                    undefined = true;
+                    return -1;
+                }
+                if (g.geo[c] == 1) return c * g.nq + q;
+                int64_t pu;
+                return member(3, c, pu) ? encode_prev(c * g.nq + q) : -1;
             }
             int64_t pc;
             if (!member(k, c, pc)) {
@@ -196,7 +203,10 @@ struct Resolver {
             }
             int64_t s;
             int qs;
-            if (!source(k, c, q, s, qs)) return -2;
+            if (!source(k, c, q, s, qs)) {
+                undefined = true;
+                return -1;
+            }
             int64_t ps;
             const bool same_phase = member(k, s, ps);
             if (same_phase && ps != pc) ++order_sensitive;
@@ -214,13 +224,18 @@ BcTable resolve_boundary(const RefGeometry &g)
 {
     BcTable t;
     Resolver r(g);
-    auto add = [&](int64_t s, int q) {
-        const int64_t v = r.value(s, q, 5);
-        if (v == -2) {
+    std::vector<char> sensitive, pulled;  // per entry
+    auto add = [&](int64_t s, int q, bool is_pulled) {
+        bool undefined = false;
+        const int64_t before = r.order_sensitive;
+        const int64_t v = r.value(s, q, 5, undefined);
+        if (undefined) {
             t.undefined = true;
             t.error = "boundary value depends on a read past the end of the reference arrays";
         }
-        t.entries.push_back(FillEntry{s * g.nq + q, v >= 0 ? v : -1});
+        t.entries.push_back(FillEntry{s * g.nq + q, v});
+        sensitive.push_back(r.order_sensitive != before);
+        pulled.push_back(is_pulled);
     };
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     const int zmax = g.dim == 3 ? nz : 0;
@@ -233,21 +248,37 @@ BcTable resolve_boundary(const RefGeometry &g)
                 if (g.geo[c] != 1) continue;
                 for (int q = 0; q < g.nq; ++q) {
                     const int sx = x - g.lat.e[q][0], sy = y - g.lat.e[q][1], sz = z - g.lat.e[q][2];
-                    if (!g.interior(sx, sy, sz)) add(c - g.eoff(q, 1), q);
+                    if (!g.interior(sx, sy, sz)) add(c - g.eoff(q, 1), q, true);
                 }
             }
         }
-    // (b) fluid nodes pull from interior body cells of the b_user list
+    // (b) every direction of every interior body cell of the b_user list:
+    // fluid nodes pull some of them now, and all of them carry over as the
+    // next step's previous values
     for (int32_t s : g.b_user) {
         int x, y, z;
         g.coords(s, x, y, z);
         if (!g.interior(x, y, z)) continue;  // ghost body cells are covered by (a)
         for (int q = 0; q < g.nq; ++q) {
             const int px = x + g.lat.e[q][0], py = y + g.lat.e[q][1], pz = z + g.lat.e[q][2];
-            if (g.interior(px, py, pz) && g.geo[g.id(px, py, pz)] == 1) add(s, q);
+            add(s, q, g.interior(px, py, pz) && g.geo[g.id(px, py, pz)] == 1);
         }
     }
-    t.order_sensitive = r.order_sensitive;
+    // Order-sensitive reads matter when they reach a fluid node: directly,
+    // or through a previous-step body value that was itself order-sensitive.
+    std::unordered_map<int64_t, size_t> at;
+    for (size_t i = 0; i < t.entries.size(); ++i) at.emplace(t.entries[i].ref_slot, i);
+    for (bool changed = true; changed;) {
+        changed = false;
+        for (size_t i = 0; i < t.entries.size(); ++i) {
+            const int64_t src = t.entries[i].ref_src;
+            if (sensitive[i] || src > -2) continue;
+            auto it = at.find(decode_prev(src));
+            if (it != at.end() && sensitive[it->second]) sensitive[i] = changed = true;
+        }
+    }
+    t.order_sensitive = 0;
+    for (size_t i = 0; i < t.entries.size(); ++i) t.order_sensitive += sensitive[i] && pulled[i];
     return t;
 }
 

@@ -50,12 +50,23 @@ struct RefGeometry {
     }
 };
 
-// One resolved slot: device slot written by the boundary fill and the
-// reference slot whose post-collision value it receives (-1: zero).
+// One resolved slot: the reference slot the boundary fill writes and where
+// its value comes from.  Encoding of ref_src:
+//   >= 0  cell*nq + q of an interior fluid slot, post-collision value of
+//         this step;
+//   -1    zero (a solid cell the reference never writes);
+//   <= -2 the value slot (-ref_src - 2) of an interior body cell held at the
+//         end of the previous step's BoundaryCondition: Collision never
+//         touches solid cells, so what the User phase wrote there persists
+//         into the next step, where a wall phase can read it through a
+//         periodic image before the User phase rewrites it.
 struct FillEntry {
     int64_t ref_slot;  // cell*nq + q in the reference layout
-    int64_t ref_src;   // cell*nq + q of an interior fluid slot, or -1
+    int64_t ref_src;
 };
+
+inline int64_t encode_prev(int64_t slot) { return -slot - 2; }
+inline int64_t decode_prev(int64_t src) { return -src - 2; }
 
 struct BcTable {
     std::vector<FillEntry> entries;
@@ -65,9 +76,12 @@ struct BcTable {
     std::string error;
 };
 
-// Resolve every slot that a fluid node pulls from a ghost cell or a body
-// (b_user) cell through PassF and the Up, Down, User, Inlet, Outlet phases
-// of BoundaryCondition (lbm.f90:96-155), each phase walked in list order.
+// Resolve, through PassF and the Up, Down, User, Inlet, Outlet phases of
+// BoundaryCondition (lbm.f90:96-155, each phase walked in list order):
+//  (a) every slot a fluid node pulls from a ghost cell, and
+//  (b) every slot of every interior body (b_user) cell -- the ones fluid
+//      nodes pull now and the ones the next step may read as its previous
+//      value.
 BcTable resolve_boundary(const RefGeometry &g);
 
 }  // namespace imexlb

@@ -54,7 +54,8 @@ hipError_t launch_step(int dim, int prec, int mode, const StepArgs &a, hipStream
 hipError_t launch_collide(int dim, int prec, const StepArgs &a, hipStream_t s);
 hipError_t launch_init_pdf(int dim, int prec, const StepArgs &a, hipStream_t s);
 hipError_t launch_init_up(int dim, const StepArgs &a, const double *rv, hipStream_t s);
-hipError_t launch_fill(int prec, void *buf, const int64_t *dslot, const int64_t *sslot, int64_t n, hipStream_t s);
+hipError_t launch_fill(int prec, void *buf, const void *prev, const int64_t *dslot, const int64_t *sslot, int64_t n,
+                       hipStream_t s);
 hipError_t launch_monitor(int dim, const StepArgs &a, uint32_t skip, unsigned long long *out, hipStream_t s);
 
 }  // namespace imexlb

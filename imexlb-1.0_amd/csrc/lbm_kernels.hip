@@ -349,15 +349,19 @@ __global__ __launch_bounds__(256) void k_init_up(double *__restrict__ p, double 
     }
 }
 
-// Boundary fill: dst slot <- resolved interior fluid slot (or 0).
+// Boundary fill of this step's buffer: dst slot <- this step's fluid slot
+// (s >= 0), 0 (s == -1), or a body-cell slot of the previous step's buffer
+// (s <= -2, slot -s-2).  Sources are never destinations, so the order of
+// the entries does not matter.
 template <class S>
-__global__ __launch_bounds__(256) void k_fill(S *__restrict__ buf, const int64_t *__restrict__ dslot,
-                                              const int64_t *__restrict__ sslot, int64_t n)
+__global__ __launch_bounds__(256) void k_fill(S *__restrict__ buf, const S *__restrict__ prev,
+                                              const int64_t *__restrict__ dslot, const int64_t *__restrict__ sslot,
+                                              int64_t n)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t s = sslot[i];
-    buf[dslot[i]] = s >= 0 ? buf[s] : Store<S>::template st<float>(0.0f);
+    buf[dslot[i]] = s >= 0 ? buf[s] : s == -1 ? Store<S>::template st<float>(0.0f) : prev[-s - 2];
 }
 
 // Monitor (lbm.f90:212-217): max of ux^2+uy^2 over fluid nodes except the
@@ -474,12 +478,14 @@ hipError_t launch_init_up(int dim, const StepArgs &a, const double *rv, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_fill(int prec, void *buf, const int64_t *dslot, const int64_t *sslot, int64_t n, hipStream_t s)
+hipError_t launch_fill(int prec, void *buf, const void *prev, const int64_t *dslot, const int64_t *sslot, int64_t n,
+                       hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     return dispatch_prec(prec, [&](auto *sp, auto *) {
         using S = std::remove_pointer_t<decltype(sp)>;
-        hipLaunchKernelGGL((k_fill<S>), dim3(nblocks(uint64_t(n))), dim3(256), 0, s, (S *)buf, dslot, sslot, n);
+        hipLaunchKernelGGL((k_fill<S>), dim3(nblocks(uint64_t(n))), dim3(256), 0, s, (S *)buf, (const S *)prev, dslot,
+                           sslot, n);
         return hipGetLastError();
     });
 }

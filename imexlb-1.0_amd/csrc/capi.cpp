@@ -11,6 +11,7 @@
 // until the next PostProcessing.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -74,6 +75,11 @@ struct imexlb_ctx {
     int64_t *fill_dst = nullptr, *fill_src = nullptr;
     int64_t nfill = 0;
     unsigned long long *red = nullptr;
+    // body links for the momentum-exchange force
+    int64_t *link_out = nullptr, *link_back = nullptr;
+    int8_t *link_q = nullptr;
+    int64_t nlink = 0;
+    double *force_partial = nullptr, *force_out = nullptr;
     size_t dev_bytes = 0;
 
     // phase state
@@ -174,7 +180,8 @@ struct imexlb_ctx {
             (void)hipEventDestroy(t.b);
         }
         timed.clear();
-        void *ptrs[] = {fb[0], fb[1], f, p, ux, uy, uz, rv, geo, fill_dst, fill_src, red};
+        void *ptrs[] = {fb[0], fb[1], f,        p,         ux,     uy,           uz,       rv,
+                        geo,   fill_dst, fill_src, red, link_out, link_back, link_q, force_partial, force_out};
         for (void *q : ptrs)
             if (q) (void)hipFree(q);
         fb[0] = fb[1] = f = nullptr;
@@ -182,6 +189,9 @@ struct imexlb_ctx {
         geo = nullptr;
         fill_dst = fill_src = nullptr;
         red = nullptr;
+        link_out = link_back = nullptr;
+        link_q = nullptr;
+        force_partial = force_out = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -453,6 +463,45 @@ int imexlb_set_geometry(imexlb_ctx *c, double pos, double radius)
         HIP_TRY(hipMemcpy(c->fill_dst, dst.data(), dst.size() * 8, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(c->fill_src, src.data(), src.size() * 8, hipMemcpyHostToDevice));
     }
+
+    // body links: fluid node x, direction q, interior solid neighbour x+e_q
+    std::vector<int64_t> lo, lb;
+    std::vector<int8_t> lq;
+    const int nzmax = c->dim == 3 ? c->kp.nz : 0;
+    for (int z = 0; z <= nzmax; ++z)
+        for (int y = 0; y <= c->kp.ny; ++y)
+            for (int x = 0; x <= c->kp.nx; ++x) {
+                if (g.geo[size_t(g.id(x, y, z))] != 0) continue;
+                for (int q = 0; q < c->Q; ++q) {
+                    const int fx = x - g.lat.e[q][0], fy = y - g.lat.e[q][1], fz = z - g.lat.e[q][2];
+                    if (!g.interior(fx, fy, fz) || g.geo[size_t(g.id(fx, fy, fz))] != 1) continue;
+                    lo.push_back(int64_t(q) * c->plane + c->pidx(fx, fy, fz));
+                    lb.push_back(int64_t(c->Q - 1 - q) * c->plane + c->pidx(x, y, z));
+                    lq.push_back(int8_t(q));
+                }
+            }
+    for (void *q : {(void *)c->link_out, (void *)c->link_back, (void *)c->link_q, (void *)c->force_partial,
+                    (void *)c->force_out})
+        if (q) (void)hipFree(q);
+    c->link_out = c->link_back = nullptr;
+    c->link_q = nullptr;
+    c->force_partial = c->force_out = nullptr;
+    c->nlink = int64_t(lo.size());
+    {
+        const size_t nl = std::max<size_t>(lo.size(), 1);
+        const size_t nblk = (nl + 255) / 256;
+        int rc;
+        if ((rc = dev_alloc(c, (void **)&c->link_out, nl * 8)) || (rc = dev_alloc(c, (void **)&c->link_back, nl * 8)) ||
+            (rc = dev_alloc(c, (void **)&c->link_q, nl)) ||
+            (rc = dev_alloc(c, (void **)&c->force_partial, nblk * 3 * sizeof(double))) ||
+            (rc = dev_alloc(c, (void **)&c->force_out, 4 * sizeof(double))))
+            return rc;
+        if (!lo.empty()) {
+            HIP_TRY(hipMemcpy(c->link_out, lo.data(), lo.size() * 8, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(c->link_back, lb.data(), lb.size() * 8, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(c->link_q, lq.data(), lq.size(), hipMemcpyHostToDevice));
+        }
+    }
     c->stage = S_NONE;
     c->have_pu = c->have_f = c->have_fb = false;
     c->lazy_src = -1;
@@ -685,6 +734,32 @@ int imexlb_monitor(imexlb_ctx *c, double *um)
     double m;
     std::memcpy(&m, &bits, sizeof(m));
     *um = std::sqrt(m);
+    return IMEXLB_OK;
+}
+
+int imexlb_forces(imexlb_ctx *c, double F[3])
+{
+    CHECK_DEV(c);
+    if (!F) return fail(IMEXLB_E_ARG, "null pointer");
+    const void *buf;
+    if (c->stage == S_FUSED)
+        buf = c->fb[c->lazy_src];
+    else if (c->have_fb && (c->stage == S_FILLED || c->stage == S_PENDING || c->stage == S_STREAMED ||
+                            c->stage == S_PRE))
+        buf = c->fb[c->cur];
+    else
+        return fail(IMEXLB_E_STATE, "forces need a step's BoundaryCondition to have run");
+    int rc = c->launch(2, [&] {
+        return launch_forces(c->dim, c->prec, buf, c->link_out, c->link_back, c->link_q, c->nlink, c->force_partial,
+                             c->force_out, c->stream);
+    });
+    if (rc) return rc;
+    double h[3];
+    HIP_TRY(hipMemcpyAsync(h, c->force_out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    F[0] = h[0];
+    F[1] = h[1];
+    F[2] = c->dim == 3 ? h[2] : 0.0;
     return IMEXLB_OK;
 }
 

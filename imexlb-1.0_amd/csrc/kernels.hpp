@@ -57,5 +57,8 @@ hipError_t launch_init_up(int dim, const StepArgs &a, const double *rv, hipStrea
 hipError_t launch_fill(int prec, void *buf, const void *prev, const int64_t *dslot, const int64_t *sslot, int64_t n,
                        hipStream_t s);
 hipError_t launch_monitor(int dim, const StepArgs &a, uint32_t skip, unsigned long long *out, hipStream_t s);
+// body-link momentum exchange; partial holds 3 doubles per 256 links (at least 3)
+hipError_t launch_forces(int dim, int prec, const void *buf, const int64_t *out_slot, const int64_t *back_slot,
+                         const int8_t *lq, int64_t n, double *partial, double *out, hipStream_t s);
 
 }  // namespace imexlb

@@ -391,6 +391,48 @@ __global__ __launch_bounds__(256) void k_monitor(const double *__restrict__ ux, 
 
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
+// Momentum exchange on the body links (fluid x -> solid x+e_q) of the
+// post-BoundaryCondition buffer: F += e_q (fb(x,q) + fb(x+e_q, opposite q)),
+// FP64.  Stage 1 writes one partial sum per block; stage 2 adds the partials
+// in block order, so the result does not depend on scheduling.
+template <int D, class S>
+__global__ __launch_bounds__(256) void k_forces(const S *__restrict__ buf, const int64_t *__restrict__ out_slot,
+                                                const int64_t *__restrict__ back_slot,
+                                                const int8_t *__restrict__ lq, int64_t n,
+                                                double *__restrict__ partial)
+{
+    using L = Lat<D>;
+    __shared__ double red[4][3];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    double s[3] = {0.0, 0.0, 0.0};
+    if (i < n) {
+        const int q = lq[i];
+        const double m = double(Store<S>::ld(buf[out_slot[i]])) + double(Store<S>::ld(buf[back_slot[i]]));
+        s[0] = double(L::ex(q)) * m;
+        s[1] = double(L::ey(q)) * m;
+        s[2] = double(L::ez(q)) * m;
+    }
+    for (int d = 0; d < 3; ++d)
+        for (int o = 32; o > 0; o >>= 1) s[d] += __shfl_xor(s[d], o, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int d = 0; d < 3; ++d) red[wave][d] = s[d];
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0.0;
+        for (int w = 0; w < int(blockDim.x >> 6); ++w) t += red[w][threadIdx.x];
+        partial[int64_t(blockIdx.x) * 3 + threadIdx.x] = t;
+    }
+}
+
+__global__ void k_forces_final(const double *__restrict__ partial, int64_t nblk, double *__restrict__ out)
+{
+    if (threadIdx.x >= 3) return;
+    double t = 0.0;
+    for (int64_t b = 0; b < nblk; ++b) t += partial[b * 3 + threadIdx.x];
+    out[threadIdx.x] = t;
+}
+
 // ---------------------------------------------------------------------------
 // host-side dispatch
 
@@ -488,6 +530,25 @@ hipError_t launch_fill(int prec, void *buf, const void *prev, const int64_t *dsl
                            sslot, n);
         return hipGetLastError();
     });
+}
+
+hipError_t launch_forces(int dim, int prec, const void *buf, const int64_t *out_slot, const int64_t *back_slot,
+                         const int8_t *lq, int64_t n, double *partial, double *out, hipStream_t s)
+{
+    const unsigned nb = n > 0 ? nblocks(uint64_t(n)) : 1u;
+    hipError_t e = dispatch_prec(prec, [&](auto *sp, auto *) {
+        using S = std::remove_pointer_t<decltype(sp)>;
+        if (dim == 2)
+            hipLaunchKernelGGL((k_forces<2, S>), dim3(nb), dim3(256), 0, s, (const S *)buf, out_slot, back_slot, lq, n,
+                               partial);
+        else
+            hipLaunchKernelGGL((k_forces<3, S>), dim3(nb), dim3(256), 0, s, (const S *)buf, out_slot, back_slot, lq, n,
+                               partial);
+        return hipGetLastError();
+    });
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_forces_final, dim3(1), dim3(64), 0, s, partial, int64_t(nb), out);
+    return hipGetLastError();
 }
 
 hipError_t launch_monitor(int dim, const StepArgs &a, uint32_t skip, unsigned long long *out, hipStream_t s)

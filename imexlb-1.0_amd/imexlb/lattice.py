@@ -127,6 +127,12 @@ class Lattice:
         check(lib().imexlb_monitor(self._h, C.byref(um)), "imexlb_monitor")
         return um.value
 
+    def forces(self):
+        """Momentum-exchange force on the body [drag, lift(, lift_z)], lattice units."""
+        F = (C.c_double * 3)()
+        check(lib().imexlb_forces(self._h, F), "imexlb_forces")
+        return np.array(F[:self.dim])
+
     def synchronize(self):
         check(lib().imexlb_synchronize(self._h), "imexlb_synchronize")
 

@@ -140,6 +140,15 @@ int imexlb_advance(imexlb_ctx *ctx, int64_t nsteps);
  * of sqrt(ux^2+uy^2).  Synchronises. */
 int imexlb_monitor(imexlb_ctx *ctx, double *um);
 
+/* Force on the body by momentum exchange, in lattice units, for the most
+ * recent BoundaryCondition: sum over fluid nodes x and directions q with an
+ * interior solid neighbour x+e_q of e_q (fb(x,q) + fb(x+e_q, nq-1-q)).
+ * F[0] is the drag (along the inlet flow), F[1] (and F[2] in 3D) the lift.
+ * No counterpart in the reference, whose Monitor (lbm.f90:205-227) logs
+ * only the maximum speed; FP64 accumulation, deterministic order.
+ * Synchronises. */
+int imexlb_forces(imexlb_ctx *ctx, double F[3]);
+
 /* TARGET UPDATE from(u,p) (lbm.f90:244): p and u of the current state. */
 int imexlb_download_fields(imexlb_ctx *ctx, double *p, double *u, int layout);
 /* f (IMEXLB_PDF_F) or fb (IMEXLB_PDF_FB) of the current state, as FP64.

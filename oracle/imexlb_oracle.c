@@ -533,6 +533,33 @@ double orc_monitor(orc *o)
 }
 
 /* ------------------------------------------------------------------------
+ * a14 Force on the body by momentum exchange (no counterpart in the
+ * reference, which logs no force).  Call after BoundaryCondition: for every
+ * fluid node x and direction q whose neighbour x+e_q is an interior solid
+ * cell, the population fb(x,q) leaves toward the body and fb(x+e_q, nq-1-q)
+ * -- the mirror value on a b_user cell, 0 on any other solid cell -- comes
+ * back:  F += e_q (fb(x,q) + fb(x+e_q, nq-1-q)).  Summed in fluid_id order,
+ * q inner, in FP64.
+ * Pinned by: F = 0 at rest, F_y = 0 for a flow symmetric about the body's
+ * centre line, and the sign of F_x for a uniform stream (tests).
+ * ---------------------------------------------------------------------- */
+void orc_forces(orc *o, double F[3])
+{
+    const int dim = o->dim, nq = o->nq;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int64_t idn = 0; idn < o->size_fluid; ++idn) {
+        int64_t id = o->fluid_id[idn];
+        for (int iq = 0; iq < nq; ++iq) {
+            int64_t nb = id + orc_eoff(o, iq, 1);
+            if (o->geo[nb] != 0 || orc_is_ghost(o, nb)) continue;
+            double m = o->fb[id * nq + iq] + o->fb[nb * nq + (nq - 1 - iq)];
+            for (int d = 0; d < dim; ++d) s[d] = s[d] + (double)o->e[iq * dim + d] * m;
+        }
+    }
+    F[0] = s[0]; F[1] = s[1]; F[2] = dim == 3 ? s[2] : 0.0;
+}
+
+/* ------------------------------------------------------------------------
  * a15 Loop orchestration (main.f90:48-63) for iterations iter0..iter1
  * inclusive: Monitor when mod(iter,interv)==0, then Collision,
  * BoundaryCondition, Propagation, PostProcessing.  Monitor values are

@@ -44,6 +44,7 @@ def lib():
         L.orc_pass_int.argtypes = [P, P]
         L.orc_monitor.restype = C.c_double
         L.orc_monitor.argtypes = [P]
+        L.orc_forces.argtypes = [P, C.POINTER(C.c_double)]
         L.orc_run.restype = C.c_int64
         L.orc_run.argtypes = [P, C.c_int64, C.c_int64, C.c_int64, C.POINTER(C.c_double), C.c_int64]
         for name in ("orc_ptr_f", "orc_ptr_fb", "orc_ptr_p", "orc_ptr_u"):
@@ -208,6 +209,12 @@ class Oracle:
 
     def monitor(self):
         return lib().orc_monitor(self._h)
+
+    def forces(self):
+        """Momentum-exchange force on the body (call after boundary_condition)."""
+        F = (C.c_double * 3)()
+        lib().orc_forces(self._h, F)
+        return np.array(F[:self.dim])
 
     def run(self, iter0, iter1, interv, cap=100000):
         buf = (C.c_double * cap)()

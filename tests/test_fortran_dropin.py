@@ -55,7 +55,7 @@ def read_plt(path, dim):
     raw = open(path, "rb").read()
     assert raw[:8] == b"#!TDV112"
     pos = 8
-    numvars = dim + 3
+    numvars = 5 if dim == 2 else 7  # coordinates + p + velocity components
     title = "Cylinder" if dim == 2 else "Sphere"
     names = ["x", "y", "p", "ux", "uy"] if dim == 2 else ["x", "y", "z", "p", "ux", "uy", "uz"]
     ints = [1, 0] + [ord(c) for c in title] + [0, numvars]

@@ -142,3 +142,19 @@ def test_precision_forms_error(precision, tol, r_vec):
     assert np.isfinite(err) and err < tol
     ref.close()
     lat.close()
+
+
+@pytest.mark.parametrize("name,steps", [("cyl400x100", 500), ("tiny3d", 40), ("inletbody2d", 100)])
+@pytest.mark.parametrize("fused", [True, False])
+def test_forces_match_oracle(name, steps, fused, r_vec):
+    """Drag and lift by momentum exchange, FP64, within 1e-12 of the
+    oracle's (the summation order differs)."""
+    lat, o, d = _run_pair(name, r_vec, steps, fused)
+    lat.advance(steps)
+    o.run(0, steps - 1, 0)
+    Fd, Fo = lat.forces(), o.forces()
+    scale = np.abs(Fo).max()
+    assert scale > 0
+    assert np.all(np.abs(Fd - Fo) <= 1e-12 * scale), (Fd, Fo)
+    lat.close()
+    o.close()

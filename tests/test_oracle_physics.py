@@ -138,3 +138,53 @@ def test_monitor_is_max_speed_skipping_first_fluid_entry():
     speeds = np.sqrt(u[fid[1:], 0] ** 2 + u[fid[1:], 1] ** 2)
     assert o.monitor() == speeds.max()
     o.close()
+
+
+def test_forces_known_answers():
+    """Momentum exchange on hand-set populations: one link to a plain solid
+    cell gives e_q * fb(x,q); a link to a body-list cell adds the value the
+    User phase mirrored into it; a lattice at rest feels no force."""
+    o = O.Oracle(2, 30, 16)
+    o.set_params(1.5, 1.0, 0.0)
+    o.set_geometry(12.0, 3.0)
+    o.boundary_condition()
+    assert np.array_equal(o.forces(), [0.0, 0.0])
+    fb = o.fb.reshape(o.A, o.nq)
+    geo = o.geo
+    users = set(o.list("b_user").tolist())
+    fid = o.list("fluid_id")
+    plain = body = None
+    for c in fid:
+        for q in range(9):
+            nb = c + o.e[q, 0] + o.dyy * o.e[q, 1]
+            if geo[nb] == 0:
+                if nb in users and body is None:
+                    body = (c, q, nb)
+                elif nb not in users and plain is None:
+                    plain = (c, q, nb)
+    c, q, nb = plain
+    fb[:] = 0.0
+    fb[c, q] = 0.25
+    assert np.array_equal(o.forces(), 0.25 * o.e[q])
+    c, q, nb = body
+    fb[:] = 0.0
+    fb[c, q] = 0.5
+    fb[nb, 8 - q] = 0.125  # what the User phase left in the body cell
+    assert np.array_equal(o.forces(), (0.5 + 0.125) * o.e[q])
+    o.close()
+
+
+def test_forces_drag_positive_in_stream(r_vec):
+    """A cylinder in the channel flow is pushed downstream."""
+    import imexlb as I
+    deck = I.deck("cyl400x100")
+    d = I.derive(deck)
+    o = O.Oracle(2, *deck.n)
+    o.set_params(deck.ome, deck.rho0, d.uu)
+    o.set_geometry(d.pos, d.radius)
+    o.init_up(r_vec)
+    o.init_pdf()
+    o.run(0, 199, 0)
+    F = o.forces()
+    assert F[0] > 0 and abs(F[1]) < F[0]
+    o.close()

@@ -20,6 +20,21 @@
 
 #include "kernels.hpp"
 
+// Build-time tuning of the step kernel: threads per block, streaming
+// stores for the post-collision output, XCD-contiguous block order.
+#ifndef IMEXLB_STEP_BLOCK
+#define IMEXLB_STEP_BLOCK 256
+#endif
+#ifndef IMEXLB_STEP_NT
+#define IMEXLB_STEP_NT 0
+#endif
+#ifndef IMEXLB_STEP_NTLOAD
+#define IMEXLB_STEP_NTLOAD 0
+#endif
+#ifndef IMEXLB_XCD_REMAP
+#define IMEXLB_XCD_REMAP 1
+#endif
+
 namespace imexlb {
 
 // ---------------------------------------------------------------------------
@@ -95,6 +110,16 @@ __device__ __forceinline__ void nt_store<bf16_t>(bf16_t *p, bf16_t v)
 {
     __builtin_nontemporal_store(v.bits, &p->bits);
 }
+template <class S>
+__device__ __forceinline__ S nt_load(const S *p)
+{
+    return __builtin_nontemporal_load(p);
+}
+template <>
+__device__ __forceinline__ bf16_t nt_load<bf16_t>(const bf16_t *p)
+{
+    return bf16_t{__builtin_nontemporal_load(&p->bits)};
+}
 
 // ---------------------------------------------------------------------------
 // node decoding
@@ -108,6 +133,7 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &d)
 // one L2.
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
 {
+    if (!IMEXLB_XCD_REMAP) return b;
     const uint32_t per = nb / 8u;
     if (b >= per * 8u) return b;
     return (b % 8u) * per + b / 8u;
@@ -217,7 +243,7 @@ __device__ __forceinline__ void collide(const C (&f)[Lat<D>::Q], C p, C ux, C uy
 // WRITE_F alone is Propagation; NOSHIFT|WRITE_PU is PostProcessing;
 // WRITE_F|WRITE_PU materialises f, p, u lazily.
 template <int D, class S, class C, int MODE>
-__global__ __launch_bounds__(256) void k_step(const S *__restrict__ src, S *__restrict__ dst,
+__global__ __launch_bounds__(IMEXLB_STEP_BLOCK) void k_step(const S *__restrict__ src, S *__restrict__ dst,
                                               S *__restrict__ fout, double *__restrict__ pout,
                                               double *__restrict__ uxo, double *__restrict__ uyo,
                                               double *__restrict__ uzo, const uint8_t *__restrict__ geo,
@@ -227,7 +253,18 @@ __global__ __launch_bounds__(256) void k_step(const S *__restrict__ src, S *__re
     int x, y, z;
     uint32_t c;
     if (!decode<D>(kp, xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, x, y, z, c)) return;
-    if (!geo[c]) {
+    // The node's type is loaded together with its populations, not before
+    // them, so the gather does not wait for it (every gathered slot is in
+    // the arrays also for solid nodes).
+    const uint8_t fluid = geo[c];
+    const uint32_t cb = c * uint32_t(sizeof(S));
+    C f[L::Q];
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+        const char *sq = (const char *)(src + (int64_t(q) * kp.plane - ((MODE & K_NOSHIFT) ? 0 : shift<D>(kp, q))));
+        f[q] = Store<S>::ld(IMEXLB_STEP_NTLOAD ? nt_load((const S *)(sq + cb)) : *(const S *)(sq + cb));
+    }
+    if (!fluid) {
         if (MODE & K_WRITE_PU) {  // PostProcessing's inlet/outlet loops also reach solid nodes
             if (x == 0) {
                 const int h = D == 2 ? y : z;
@@ -239,15 +276,10 @@ __global__ __launch_bounds__(256) void k_step(const S *__restrict__ src, S *__re
         }
         return;
     }
-    C f[L::Q];
-#pragma unroll
-    for (int q = 0; q < L::Q; ++q) {
-        const S *sq = src + (int64_t(q) * kp.plane - ((MODE & K_NOSHIFT) ? 0 : shift<D>(kp, q)));
-        f[q] = Store<S>::ld(sq[c]);
-    }
     if (MODE & K_WRITE_F) {
 #pragma unroll
-        for (int q = 0; q < L::Q; ++q) fout[int64_t(q) * kp.plane + c] = Store<S>::template st<C>(f[q]);
+        for (int q = 0; q < L::Q; ++q)
+            *(S *)((char *)(fout + int64_t(q) * kp.plane) + cb) = Store<S>::template st<C>(f[q]);
     }
     if (!(MODE & (K_WRITE_PU | K_COLLIDE))) return;
     C p, ux, uy, uz;
@@ -260,7 +292,11 @@ __global__ __launch_bounds__(256) void k_step(const S *__restrict__ src, S *__re
     }
     if (MODE & K_COLLIDE) {
         collide<D, C>(f, p, ux, uy, uz, kp, [&](int q, C v) {
-            nt_store(dst + int64_t(q) * kp.plane + c, Store<S>::template st<C>(v));
+            S *o = (S *)((char *)(dst + int64_t(q) * kp.plane) + cb);
+            if (IMEXLB_STEP_NT)
+                nt_store(o, Store<S>::template st<C>(v));
+            else
+                *o = Store<S>::template st<C>(v);
         });
     }
 }
@@ -456,7 +492,8 @@ static hipError_t step_dm(int prec, const StepArgs &a, hipStream_t s)
     return dispatch_prec(prec, [&](auto *sp, auto *cp) {
         using S = std::remove_pointer_t<decltype(sp)>;
         using C = std::remove_pointer_t<decltype(cp)>;
-        hipLaunchKernelGGL((k_step<D, S, C, MODE>), dim3(nblocks(a.kp.N)), dim3(256), 0, s, (const S *)a.src,
+        const unsigned nb = unsigned((uint64_t(a.kp.N) + IMEXLB_STEP_BLOCK - 1) / IMEXLB_STEP_BLOCK);
+        hipLaunchKernelGGL((k_step<D, S, C, MODE>), dim3(nb), dim3(IMEXLB_STEP_BLOCK), 0, s, (const S *)a.src,
                            (S *)a.dst, (S *)a.fout, a.p, a.ux, a.uy, a.uz, a.geo, a.kp);
         return hipGetLastError();
     });

@@ -328,6 +328,7 @@ void orc_init_up(orc *o, const double *r_vec)
     const int dim = o->dim;
     const int ny = o->n[1], nz = o->n[2];
     int kmax = dim == 3 ? o->L[2] : 1;
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < kmax; ++k)
         for (int j = 0; j < o->L[1]; ++j)
             for (int i = 0; i < o->L[0]; ++i) {

@@ -37,14 +37,25 @@ contains
   end subroutine PostProcessing
 
   ! Monitor (lbm.f90:205-227): maximum |(ux, uy)| over the fluid, printed.
+  ! With IMEXLB_FORCES=1 it also prints the momentum-exchange drag and lift
+  ! (lattice units) and their coefficients F / (rho0 uu^2 radius), which
+  ! the reference does not compute; they need a completed step.
   subroutine Monitor
-    double precision um_global
+    double precision um_global, F(3)
+    character(len=8) :: v
+    integer st
     call imx_check(imexlb_monitor(imx_ctx, um), 'Monitor')
     call MPI_BARRIER(CART_COMM, ierr)
     call MPI_REDUCE(um, um_global, 1, MPI_DOUBLE_PRECISION, MPI_MAX, 0, CART_COMM, ierr)
     if (rank == 0) then
        write(*, *) "T=", iter / t_intv
        write(*, *) "Umax=", um_global / (charlength * t_intv)
+       call get_environment_variable('IMEXLB_FORCES', v, status=st)
+       if (st == 0 .and. trim(v) == '1' .and. iter > 0) then
+          call imx_check(imexlb_forces(imx_ctx, F), 'Monitor forces')
+          write(*, *) "Drag=", F(1), " Lift=", F(2)
+          write(*, *) "Cd=", F(1) / (rho0 * uu**2 * radius), " Cl=", F(2) / (rho0 * uu**2 * radius)
+       endif
        write(*, *) "-------------------"
     endif
   end subroutine Monitor

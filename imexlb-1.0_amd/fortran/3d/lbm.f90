@@ -33,15 +33,27 @@ contains
     call imx_check(imexlb_post_processing(imx_ctx), 'PostProcessing')
   end subroutine PostProcessing
 
-  ! Monitor (lbm_3d.f90:207-229): maximum |(ux, uy)| over the fluid.
+  ! Monitor (lbm_3d.f90:207-229): maximum |(ux, uy)| over the fluid.  With
+  ! IMEXLB_FORCES=1 also the momentum-exchange force (lattice units) and the
+  ! drag coefficient F_x / (0.5 rho0 uu^2 pi radius^2), which the reference
+  ! does not compute; they need a completed step.
   subroutine Monitor
-    double precision um_global
+    double precision um_global, F(3)
+    double precision, parameter :: pi = 3.14159265358979323846d0
+    character(len=8) :: v
+    integer st
     call imx_check(imexlb_monitor(imx_ctx, um), 'Monitor')
     call MPI_BARRIER(CART_COMM, ierr)
     call MPI_REDUCE(um, um_global, 1, MPI_DOUBLE_PRECISION, MPI_MAX, 0, CART_COMM, ierr)
     if (rank == 0) then
        write(*, *) "T=", iter / t_intv
        write(*, *) "Umax=", um_global / (charlength * t_intv)
+       call get_environment_variable('IMEXLB_FORCES', v, status=st)
+       if (st == 0 .and. trim(v) == '1' .and. iter > 0) then
+          call imx_check(imexlb_forces(imx_ctx, F), 'Monitor forces')
+          write(*, *) "Fx=", F(1), " Fy=", F(2), " Fz=", F(3)
+          write(*, *) "Cd=", F(1) / (0.5d0 * rho0 * uu**2 * pi * radius**2)
+       endif
        write(*, *) "-------------------"
     endif
   end subroutine Monitor

@@ -92,6 +92,12 @@ module imexlb_capi
        real(c_double) :: um
        integer(c_int) :: rc
      end function
+     function imexlb_forces(ctx, F) bind(C, name="imexlb_forces") result(rc)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: F(3)
+       integer(c_int) :: rc
+     end function
      function imexlb_download_fields(ctx, p, u, layout) bind(C, name="imexlb_download_fields") result(rc)
        import :: c_ptr, c_int, c_double
        type(c_ptr), value :: ctx

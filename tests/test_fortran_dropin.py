@@ -101,6 +101,28 @@ def _run_driver(variant, deck_text, workdir, env_extra=None):
 
 
 @pytest.mark.gpu
+def test_driver_prints_forces(r_vec, tmp_path):
+    """IMEXLB_FORCES=1 adds drag and lift to the Monitor lines; they equal the
+    oracle's momentum exchange after the same number of steps."""
+    deck = get_deck("tiny2d")
+    stdout = _run_driver("2d", TINY["tiny2d"][1], str(tmp_path), {"IMEXLB_FORCES": "1"})
+    d = I.derive(deck)
+    drag = [float(v) for v in re.findall(r"Drag=\s*(\S+)", stdout)]
+    lift = [float(v) for v in re.findall(r"Lift=\s*(\S+)", stdout)]
+    o, _ = oracle_setup(deck, r_vec)
+    expect = []
+    for k in range(1, len(drag) + 1):
+        o.run((k - 1) * d.interv, k * d.interv - 1, 0)
+        expect.append(o.forces())
+    expect = np.array(expect)
+    assert len(drag) == (d.max_step // d.interv) and len(drag) > 3
+    scale = np.abs(expect).max()
+    assert np.all(np.abs(np.array(drag) - expect[:, 0]) <= 1e-12 * scale)
+    assert np.all(np.abs(np.array(lift) - expect[:, 1]) <= 1e-12 * scale)
+    o.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("case,variant,env", [("tiny2d", "2d", {}), ("tiny2d", "2d", {"IMEXLB_FUSED": "0"}),
                                               ("tiny3d", "3d", {})])
 def test_driver_matches_oracle(case, variant, env, r_vec, tmp_path):

@@ -89,6 +89,40 @@ def test_graph_replay_matches_oracle(r_vec):
     o.close()
 
 
+def test_warm_start_hook(r_vec):
+    """Warm start (InitUP/InitPDF replaced by a given field, lbm.f90:24-48):
+    a developed p, u uploaded into a fresh lattice, InitPDF, 200 steps, equals
+    the oracle started from the same field, bit for bit; both layouts."""
+    deck = get_deck("cyl400x100")
+    d = I.derive(deck)
+    src, _ = oracle_setup(deck, r_vec)
+    src.run(0, 299, 0)  # a developed field
+    p_ref, u_ref = src.p.copy(), src.u.copy()
+    for layout in (I.LAYOUT_REF, I.LAYOUT_INTERIOR):
+        lat = I.Lattice(2, *deck.n[:2], device=0)
+        lat.set_params(deck.ome, deck.rho0, d.uu)
+        lat.set_geometry(d.pos, d.radius)
+        if layout == I.LAYOUT_REF:
+            lat.upload_fields(p_ref, u_ref, layout)
+        else:
+            p_in = src.interior(p_ref, 1)[..., 0].ravel()
+            u_in = src.interior(u_ref, 2)
+            lat.upload_fields(p_in, np.concatenate([u_in[..., 0].ravel(), u_in[..., 1].ravel()]), layout)
+        lat.init_pdf()
+        o = __import__("oracle").Oracle(2, *deck.n[:2])
+        o.set_params(deck.ome, deck.rho0, d.uu)
+        o.set_geometry(d.pos, d.radius)
+        o.p[:] = p_ref
+        o.u[:] = u_ref
+        o.init_pdf()
+        lat.advance(200)
+        o.run(0, 199, 0)
+        assert_state_equal(lat, o)
+        lat.close()
+        o.close()
+    src.close()
+
+
 def test_phase_by_phase(r_vec):
     """State after each of Collision, BoundaryCondition, Propagation and
     PostProcessing (unfused), and after the fused step, equals the oracle's."""

@@ -348,10 +348,9 @@ int imexlb_create(imexlb_ctx **out, int dim, int nx, int ny, int nz, int device,
     c->Pz = dim == 3 ? int64_t(nz) + 3 : 1;
     c->PxPy = c->Px * c->Py;
     c->o0 = 1 + c->Px + (dim == 3 ? c->PxPy : 0);
-    c->plane = ((c->Px * c->Py * c->Pz + 63) / 64) * 64;
+    c->plane = ((c->Px * c->Py * c->Pz + 127) / 128) * 128;  // whole tiles in the tiled layout
     const int64_t nodes = int64_t(nx + 1) * (ny + 1) * (dim == 3 ? nz + 1 : 1);
-    if (c->plane >= (int64_t(1) << 31) || nodes >= (int64_t(1) << 31) ||
-        c->plane * int64_t(c->ssz) >= (int64_t(1) << 32))
+    if (c->plane >= (int64_t(1) << 31) || nodes >= (int64_t(1) << 31))
         return fail(IMEXLB_E_ARG, "grid too large for 32-bit node indexing");
 
     KParams &k = c->kp;
@@ -371,6 +370,13 @@ int imexlb_create(imexlb_ctx **out, int dim, int nx, int ny, int nz, int device,
     k.Px = c->Px;
     k.PxPy = dim == 3 ? c->PxPy : 0;
     k.plane = c->plane;
+    k.ncell = uint32_t(c->Px * c->Py * c->Pz);
+    k.Py = c->Py;
+    k.div_px.init(uint32_t(c->Px));
+    k.div_py.init(uint32_t(c->Py));
+    for (uint32_t n : {0u, uint32_t(c->Px), k.ncell - 1u, 0xfffffffeu})
+        if (k.div_px.host_div(n) != n / uint32_t(c->Px) || k.div_py.host_div(n) != n / uint32_t(c->Py))
+            return fail(IMEXLB_E_ARG, "fast division self-check failed");
     k.rdy = c->g.dy;
     k.rdz = c->g.dz;
 
@@ -461,7 +467,7 @@ int imexlb_set_geometry(imexlb_ctx *c, double pos, double radius)
     auto dev_slot = [&](int64_t ref_slot) {
         int x, y, z;
         g.coords(ref_slot / c->Q, x, y, z);
-        return int64_t(ref_slot % c->Q) * c->plane + c->pidx(x, y, z);
+        return pop_slot(ref_slot % c->Q, c->pidx(x, y, z), c->plane, c->Q);
     };
     const int64_t nslots = int64_t(c->Q) * c->plane;
     for (const FillEntry &e : c->bc.entries) {
@@ -499,8 +505,8 @@ int imexlb_set_geometry(imexlb_ctx *c, double pos, double radius)
                 for (int q = 0; q < c->Q; ++q) {
                     const int fx = x - g.lat.e[q][0], fy = y - g.lat.e[q][1], fz = z - g.lat.e[q][2];
                     if (!g.interior(fx, fy, fz) || g.geo[size_t(g.id(fx, fy, fz))] != 1) continue;
-                    lo.push_back(int64_t(q) * c->plane + c->pidx(fx, fy, fz));
-                    lb.push_back(int64_t(c->Q - 1 - q) * c->plane + c->pidx(x, y, z));
+                    lo.push_back(pop_slot(q, c->pidx(fx, fy, fz), c->plane, c->Q));
+                    lb.push_back(pop_slot(c->Q - 1 - q, c->pidx(x, y, z), c->plane, c->Q));
                     lq.push_back(int8_t(q));
                 }
             }
@@ -896,26 +902,25 @@ int imexlb_download_pdf(imexlb_ctx *c, int which, double *out, int layout)
     }
     const int Q = c->Q;
     const int64_t N = c->kp.N;
-    std::vector<uint8_t> h(size_t(c->plane) * c->ssz);
+    // the whole population array, then every direction of every interior node
+    std::vector<uint8_t> h(size_t(Q) * size_t(c->plane) * c->ssz);
+    HIP_TRY(hipMemcpyAsync(h.data(), src, h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     for (int q = 0; q < Q; ++q) {
-        HIP_TRY(hipMemcpyAsync(h.data(), (const uint8_t *)src + size_t(q) * c->plane * c->ssz, h.size(),
-                               hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
         for_interior(c, [&](int, int, int, int64_t n, int64_t rid, int64_t pi) {
-            const double v = load_value(h.data(), size_t(pi), c->prec);
+            const double v = load_value(h.data(), size_t(pop_slot(q, pi, c->plane, Q)), c->prec);
             if (layout == IMEXLB_LAYOUT_REF)
                 out[rid * Q + q] = v;
             else
                 out[int64_t(q) * N + n] = v;
         });
-        if (which == IMEXLB_PDF_FB && layout == IMEXLB_LAYOUT_REF) {
-            for (const FillEntry &e : c->bc.entries) {
-                if (int(e.ref_slot % Q) != q) continue;
-                const int64_t cell = e.ref_slot / Q;
-                int x, y, z;
-                c->g.coords(cell, x, y, z);
-                out[e.ref_slot] = load_value(h.data(), size_t(c->pidx(x, y, z)), c->prec);
-            }
+    }
+    if (which == IMEXLB_PDF_FB && layout == IMEXLB_LAYOUT_REF) {
+        for (const FillEntry &e : c->bc.entries) {
+            int x, y, z;
+            c->g.coords(e.ref_slot / Q, x, y, z);
+            out[e.ref_slot] = load_value(h.data(), size_t(pop_slot(e.ref_slot % Q, c->pidx(x, y, z), c->plane, Q)),
+                                         c->prec);
         }
     }
     return IMEXLB_OK;

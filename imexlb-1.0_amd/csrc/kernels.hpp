@@ -25,6 +25,12 @@ struct FastDiv {
     }
 };
 
+// Population layout: 0 = one plane per direction; T > 0 = tiles of T padded
+// cells, all directions of a tile contiguous (slot ((c/T)*Q + q)*T + c%T).
+#ifndef IMEXLB_TILE
+#define IMEXLB_TILE 0
+#endif
+
 // Geometry and scalars every kernel needs (passed by value).
 struct KParams {
     int nx, ny, nz;      // largest interior node index per axis
@@ -36,7 +42,23 @@ struct KParams {
     int64_t plane;       // values per population plane
     int64_t rdy, rdz;    // reference-layout strides (InitUP perturbation index)
     double ome, rho0, uu;
+    uint32_t ncell;      // padded cells Px*Py*Pz
+    int64_t Py;          // padded rows per z plane
+    FastDiv div_px;      // / Px
+    FastDiv div_py;      // / Py
 };
+
+// slot of direction q at padded cell c, host side (kernels use pslot)
+inline int64_t pop_slot(int64_t q, int64_t c, int64_t plane, int Q)
+{
+#if IMEXLB_TILE
+    (void)plane;
+    return ((c / IMEXLB_TILE) * Q + q) * IMEXLB_TILE + c % IMEXLB_TILE;
+#else
+    (void)Q;
+    return q * plane + c;
+#endif
+}
 
 enum : int { K_COLLIDE = 1, K_WRITE_F = 2, K_WRITE_PU = 4, K_NOSHIFT = 8 };
 

@@ -34,6 +34,10 @@
 #ifndef IMEXLB_XCD_REMAP
 #define IMEXLB_XCD_REMAP 1
 #endif
+// thread i handles padded cell i (1) or interior node i (0)
+#ifndef IMEXLB_CELLMAP
+#define IMEXLB_CELLMAP 0
+#endif
 
 namespace imexlb {
 
@@ -157,11 +161,50 @@ __device__ __forceinline__ bool decode(const KParams &kp, uint32_t i, int &x, in
     return true;
 }
 
+// Cell-mapped threads: thread i is padded cell c = i; threads on ghost or
+// padding cells leave.  Waves then cover whole tiles of the tiled layout.
+template <int D>
+__device__ __forceinline__ bool decode_cell(const KParams &kp, uint32_t i, int &x, int &y, int &z, uint32_t &c)
+{
+    if (i >= kp.ncell) return false;
+    const uint32_t r = fdiv(i, kp.div_px);
+    x = int(i - r * uint32_t(kp.Px)) - 1;
+    if (D == 3) {
+        const uint32_t zz = fdiv(r, kp.div_py);
+        y = int(r - zz * uint32_t(kp.Py)) - 1;
+        z = int(zz) - 1;
+    } else {
+        y = int(r) - 1;
+        z = 0;
+    }
+    c = i;
+    return x >= 0 && x <= kp.nx && y >= 0 && y <= kp.ny && (D == 2 || (z >= 0 && z <= kp.nz));
+}
+
+// node of thread i in the configured thread mapping
+template <int D>
+__device__ __forceinline__ bool node_of(const KParams &kp, uint32_t i, int &x, int &y, int &z, uint32_t &c)
+{
+    return IMEXLB_CELLMAP ? decode_cell<D>(kp, i, x, y, z, c) : decode<D>(kp, i, x, y, z, c);
+}
+
 template <int D>
 __device__ __forceinline__ int64_t shift(const KParams &kp, int q)
 {
     using L = Lat<D>;
     return int64_t(L::ex(q)) + kp.Px * L::ey(q) + kp.PxPy * L::ez(q);
+}
+
+// Population slot of direction q at padded cell c: planes (one per q) or
+// tiles of IMEXLB_TILE cells holding all directions of the tile together.
+template <int Q>
+__device__ __forceinline__ int64_t pslot(const KParams &kp, int q, int64_t c)
+{
+#if IMEXLB_TILE
+    return ((c / IMEXLB_TILE) * Q + q) * IMEXLB_TILE + (c % IMEXLB_TILE);
+#else
+    return int64_t(q) * kp.plane + c;
+#endif
 }
 
 // ---------------------------------------------------------------------------
@@ -252,17 +295,16 @@ __global__ __launch_bounds__(IMEXLB_STEP_BLOCK) void k_step(const S *__restrict_
     using L = Lat<D>;
     int x, y, z;
     uint32_t c;
-    if (!decode<D>(kp, xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, x, y, z, c)) return;
+    if (!node_of<D>(kp, xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, x, y, z, c)) return;
     // The node's type is loaded together with its populations, not before
     // them, so the gather does not wait for it (every gathered slot is in
     // the arrays also for solid nodes).
     const uint8_t fluid = geo[c];
-    const uint32_t cb = c * uint32_t(sizeof(S));
     C f[L::Q];
 #pragma unroll
     for (int q = 0; q < L::Q; ++q) {
-        const char *sq = (const char *)(src + (int64_t(q) * kp.plane - ((MODE & K_NOSHIFT) ? 0 : shift<D>(kp, q))));
-        f[q] = Store<S>::ld(IMEXLB_STEP_NTLOAD ? nt_load((const S *)(sq + cb)) : *(const S *)(sq + cb));
+        const S *a = src + pslot<L::Q>(kp, q, int64_t(c) - ((MODE & K_NOSHIFT) ? 0 : shift<D>(kp, q)));
+        f[q] = Store<S>::ld(IMEXLB_STEP_NTLOAD ? nt_load(a) : *a);
     }
     if (!fluid) {
         if (MODE & K_WRITE_PU) {  // PostProcessing's inlet/outlet loops also reach solid nodes
@@ -278,8 +320,7 @@ __global__ __launch_bounds__(IMEXLB_STEP_BLOCK) void k_step(const S *__restrict_
     }
     if (MODE & K_WRITE_F) {
 #pragma unroll
-        for (int q = 0; q < L::Q; ++q)
-            *(S *)((char *)(fout + int64_t(q) * kp.plane) + cb) = Store<S>::template st<C>(f[q]);
+        for (int q = 0; q < L::Q; ++q) fout[pslot<L::Q>(kp, q, c)] = Store<S>::template st<C>(f[q]);
     }
     if (!(MODE & (K_WRITE_PU | K_COLLIDE))) return;
     C p, ux, uy, uz;
@@ -292,7 +333,7 @@ __global__ __launch_bounds__(IMEXLB_STEP_BLOCK) void k_step(const S *__restrict_
     }
     if (MODE & K_COLLIDE) {
         collide<D, C>(f, p, ux, uy, uz, kp, [&](int q, C v) {
-            S *o = (S *)((char *)(dst + int64_t(q) * kp.plane) + cb);
+            S *o = dst + pslot<L::Q>(kp, q, c);
             if (IMEXLB_STEP_NT)
                 nt_store(o, Store<S>::template st<C>(v));
             else
@@ -312,13 +353,13 @@ __global__ __launch_bounds__(256) void k_collide(const S *__restrict__ fin, S *_
     using L = Lat<D>;
     int x, y, z;
     uint32_t c;
-    if (!decode<D>(kp, xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, x, y, z, c)) return;
+    if (!node_of<D>(kp, xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, x, y, z, c)) return;
     if (!geo[c]) return;
     C f[L::Q];
 #pragma unroll
-    for (int q = 0; q < L::Q; ++q) f[q] = Store<S>::ld(fin[int64_t(q) * kp.plane + c]);
+    for (int q = 0; q < L::Q; ++q) f[q] = Store<S>::ld(fin[pslot<L::Q>(kp, q, c)]);
     collide<D, C>(f, C(p[c]), C(ux[c]), C(uy[c]), D == 3 ? C(uz[c]) : C(0), kp,
-                  [&](int q, C v) { dst[int64_t(q) * kp.plane + c] = Store<S>::template st<C>(v); });
+                  [&](int q, C v) { dst[pslot<L::Q>(kp, q, c)] = Store<S>::template st<C>(v); });
 }
 
 // InitPDF (lbm.f90:29-47): f = feq(p, u) on fluid nodes.
@@ -331,7 +372,7 @@ __global__ __launch_bounds__(256) void k_init_pdf(S *__restrict__ fout, const do
     using L = Lat<D>;
     int x, y, z;
     uint32_t c;
-    if (!decode<D>(kp, xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, x, y, z, c)) return;
+    if (!node_of<D>(kp, xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, x, y, z, c)) return;
     if (!geo[c]) return;
     const C pp = C(p[c]), u0 = C(ux[c]), u1 = C(uy[c]), u2 = D == 3 ? C(uz[c]) : C(0);
     C udu = u0 * u0 + u1 * u1;
@@ -354,7 +395,7 @@ __global__ __launch_bounds__(256) void k_init_pdf(S *__restrict__ fout, const do
         term(L::ey(q), u1);
         if (D == 3) term(L::ez(q), u2);
         const C feq = C(L::w(q)) * (pp * C(3) + rho0 * ((C(3) * edu + C(4.5) * (edu * edu)) - C(1.5) * udu));
-        fout[int64_t(q) * kp.plane + c] = Store<S>::template st<C>(feq);
+        fout[pslot<L::Q>(kp, q, c)] = Store<S>::template st<C>(feq);
     }
 }
 
@@ -473,6 +514,8 @@ __global__ void k_forces_final(const double *__restrict__ partial, int64_t nblk,
 // host-side dispatch
 
 static inline unsigned nblocks(uint64_t n) { return unsigned((n + 255) / 256); }
+// threads of the kernels that use node_of()
+static inline uint64_t node_threads(const KParams &kp) { return IMEXLB_CELLMAP ? uint64_t(kp.ncell) : uint64_t(kp.N); }
 
 template <class F>
 static hipError_t dispatch_prec(int prec, F &&f)
@@ -492,7 +535,7 @@ static hipError_t step_dm(int prec, const StepArgs &a, hipStream_t s)
     return dispatch_prec(prec, [&](auto *sp, auto *cp) {
         using S = std::remove_pointer_t<decltype(sp)>;
         using C = std::remove_pointer_t<decltype(cp)>;
-        const unsigned nb = unsigned((uint64_t(a.kp.N) + IMEXLB_STEP_BLOCK - 1) / IMEXLB_STEP_BLOCK);
+        const unsigned nb = unsigned((node_threads(a.kp) + IMEXLB_STEP_BLOCK - 1) / IMEXLB_STEP_BLOCK);
         hipLaunchKernelGGL((k_step<D, S, C, MODE>), dim3(nb), dim3(IMEXLB_STEP_BLOCK), 0, s, (const S *)a.src,
                            (S *)a.dst, (S *)a.fout, a.p, a.ux, a.uy, a.uz, a.geo, a.kp);
         return hipGetLastError();
@@ -523,7 +566,7 @@ hipError_t launch_collide(int dim, int prec, const StepArgs &a, hipStream_t s)
         return dispatch_prec(prec, [&](auto *sp, auto *cp) {
             using S = std::remove_pointer_t<decltype(sp)>;
             using C = std::remove_pointer_t<decltype(cp)>;
-            hipLaunchKernelGGL((k_collide<D, S, C>), dim3(nblocks(a.kp.N)), dim3(256), 0, s, (const S *)a.src,
+            hipLaunchKernelGGL((k_collide<D, S, C>), dim3(nblocks(node_threads(a.kp))), dim3(256), 0, s, (const S *)a.src,
                                (S *)a.dst, a.p, a.ux, a.uy, a.uz, a.geo, a.kp);
             return hipGetLastError();
         });
@@ -538,7 +581,7 @@ hipError_t launch_init_pdf(int dim, int prec, const StepArgs &a, hipStream_t s)
         return dispatch_prec(prec, [&](auto *sp, auto *cp) {
             using S = std::remove_pointer_t<decltype(sp)>;
             using C = std::remove_pointer_t<decltype(cp)>;
-            hipLaunchKernelGGL((k_init_pdf<D, S, C>), dim3(nblocks(a.kp.N)), dim3(256), 0, s, (S *)a.fout, a.p,
+            hipLaunchKernelGGL((k_init_pdf<D, S, C>), dim3(nblocks(node_threads(a.kp))), dim3(256), 0, s, (S *)a.fout, a.p,
                                a.ux, a.uy, a.uz, a.geo, a.kp);
             return hipGetLastError();
         });

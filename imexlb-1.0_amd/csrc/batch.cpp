@@ -1,0 +1,467 @@
+// batch.cpp -- batched ensemble: many independent D2Q9 lattices of their own
+// size and parameters advanced together.  Each case keeps the single-lattice
+// layout and boundary tables (built by a host-only context); the cases are
+// concatenated in one set of device arrays, so one boundary-fill launch and
+// one step launch advance all of them.  The step kernel takes the cases'
+// nodes in tiles of block*cells_per_thread nodes, by block index or from a
+// queue (persistent blocks); imexlb_batch_tune times the configurations and
+// keeps the fastest.  Pairs of steps replay as a HIP graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "context.hpp"
+
+struct imexlb_batch {
+    int ncase = 0, prec = 0, device = 0;
+    size_t ssz = 8;
+    std::vector<imexlb_ctx *> host;  // host-only contexts: geometry, tables, layout
+    std::vector<BatchCase> cases;
+    std::vector<int64_t> link_base, link_count;
+    int64_t pop_total = 0, cell_total = 0, node_total = 0, max_links = 0;
+
+    hipStream_t stream = nullptr;
+    void *fb[2] = {nullptr, nullptr};
+    void *f = nullptr;
+    double *p = nullptr, *ux = nullptr, *uy = nullptr, *rv = nullptr;
+    uint8_t *geo = nullptr;
+    int64_t *fill_dst = nullptr, *fill_src = nullptr;
+    int64_t nfill = 0;
+    int64_t *link_out = nullptr, *link_back = nullptr;
+    int8_t *link_q = nullptr;
+    double *force_partial = nullptr, *force_out = nullptr;
+    BatchCase *d_cases = nullptr;
+    uint32_t *queue = nullptr;
+    uint32_t *tile_case = nullptr, *tile_first = nullptr;
+    uint32_t ntiles = 0;
+    size_t dev_bytes = 0;
+
+    int block = 256, cpt = 1, persist = 0;
+    unsigned persist_grid = 1024;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+
+    int cur = 1, lazy = -1;
+    bool initialized = false, have_pu = false;
+
+    void drop_graphs()
+    {
+        for (auto &g : graph)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+    }
+
+    void release()
+    {
+        if (stream) {
+            (void)hipSetDevice(device);
+            (void)hipStreamSynchronize(stream);
+        }
+        drop_graphs();
+        void *ptrs[] = {fb[0],     fb[1],  f,           p,         ux,        uy,     rv,   geo,       fill_dst,
+                        fill_src,  link_out, link_back, link_q,    force_partial, force_out, d_cases, queue,
+                        tile_case, tile_first};
+        for (void *q : ptrs)
+            if (q) (void)hipFree(q);
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+        for (imexlb_ctx *c : host) imexlb_destroy(c);
+        host.clear();
+    }
+
+    // one case's kernel arguments, pointers offset to the case's arrays
+    StepArgs case_args(int i) const
+    {
+        StepArgs a;
+        a.kp = cases[size_t(i)].kp;
+        const int64_t cb = cases[size_t(i)].geo_base;
+        a.p = p + cb;
+        a.ux = ux + cb;
+        a.uy = uy + cb;
+        a.uz = nullptr;
+        a.geo = geo + cb;
+        return a;
+    }
+    void *pop(void *base, int i) const { return (uint8_t *)base + size_t(cases[size_t(i)].pop_base) * ssz; }
+
+    int build_tiles()
+    {
+        const uint32_t T = uint32_t(block * cpt);
+        std::vector<uint32_t> tc, tf;
+        for (int i = 0; i < ncase; ++i)
+            for (uint32_t first = 0; first < cases[size_t(i)].kp.N; first += T) {
+                tc.push_back(uint32_t(i));
+                tf.push_back(first);
+            }
+        if (tile_case) (void)hipFree(tile_case);
+        if (tile_first) (void)hipFree(tile_first);
+        tile_case = tile_first = nullptr;
+        ntiles = uint32_t(tc.size());
+        HIP_TRY(hipMalloc((void **)&tile_case, std::max<size_t>(tc.size(), 1) * 4));
+        HIP_TRY(hipMalloc((void **)&tile_first, std::max<size_t>(tf.size(), 1) * 4));
+        if (!tc.empty()) {
+            HIP_TRY(hipMemcpy(tile_case, tc.data(), tc.size() * 4, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(tile_first, tf.data(), tf.size() * 4, hipMemcpyHostToDevice));
+        }
+        return IMEXLB_OK;
+    }
+
+    hipError_t enqueue_step(int b)
+    {
+        hipError_t e = launch_batch_fill(prec, fb[b], fb[1 - b], fill_dst, fill_src, nfill, queue, stream);
+        if (e != hipSuccess) return e;
+        return launch_batch_step(prec, block, cpt, persist, persist_grid, fb[b], fb[1 - b], geo, d_cases, tile_case,
+                                 tile_first, ntiles, queue, stream);
+    }
+
+    int one_step()
+    {
+        hipError_t e = enqueue_step(cur);
+        if (e != hipSuccess) return fail(IMEXLB_E_HIP, std::string("batch step: ") + hipGetErrorString(e));
+        lazy = cur;
+        cur = 1 - cur;
+        have_pu = false;
+        return IMEXLB_OK;
+    }
+
+    int pair_graph(hipGraphExec_t *out)
+    {
+        if (graph[cur]) {
+            *out = graph[cur];
+            return IMEXLB_OK;
+        }
+        HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = enqueue_step(cur);
+        if (e == hipSuccess) e = enqueue_step(1 - cur);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(stream, &g);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail(IMEXLB_E_HIP, std::string("batch graph capture: ") + hipGetErrorString(e ? e : e2));
+        }
+        hipGraphExec_t x = nullptr;
+        e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(IMEXLB_E_HIP, std::string("batch graph: ") + hipGetErrorString(e));
+        graph[cur] = x;
+        *out = x;
+        return IMEXLB_OK;
+    }
+
+    // f, p, u of case i from the buffer the last step streamed from
+    int materialize(int i)
+    {
+        if (have_pu) return IMEXLB_OK;
+        if (lazy < 0) return fail(IMEXLB_E_STATE, "batch fields requested before imexlb_batch_init");
+        StepArgs a = case_args(i);
+        a.src = pop(fb[lazy], i);
+        a.fout = pop(f, i);
+        hipError_t e = launch_step(2, prec, K_WRITE_F | K_WRITE_PU, a, stream);
+        if (e != hipSuccess) return fail(IMEXLB_E_HIP, std::string("batch materialize: ") + hipGetErrorString(e));
+        return IMEXLB_OK;
+    }
+};
+
+namespace {
+
+int balloc(imexlb_batch *b, void **ptr, size_t bytes)
+{
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return fail(e == hipErrorOutOfMemory ? IMEXLB_E_OOM : IMEXLB_E_HIP, "batch hipMalloc failed");
+    }
+    b->dev_bytes += bytes;
+    e = hipMemset(*ptr, 0, bytes);
+    if (e != hipSuccess) return fail(IMEXLB_E_HIP, "batch hipMemset failed");
+    return IMEXLB_OK;
+}
+
+#define CHECK_BATCH(b)                                                    \
+    do {                                                                  \
+        if (!(b)) return fail(IMEXLB_E_ARG, "null batch");               \
+        HIP_TRY(hipSetDevice((b)->device));                               \
+    } while (0)
+
+}  // namespace
+
+extern "C" {
+
+int imexlb_batch_create(imexlb_batch **out, int ncase, const int32_t *nxny, const double *params, int device,
+                        int precision)
+{
+    if (!out || !nxny || !params || ncase <= 0) return fail(IMEXLB_E_ARG, "batch arguments");
+    *out = nullptr;
+    if (precision < IMEXLB_F64 || precision > IMEXLB_F16_F32)
+        return fail(IMEXLB_E_ARG, "batch precision must be FP64, FP32 or FP16 storage");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(IMEXLB_E_ARG, "device index out of range");
+    std::unique_ptr<imexlb_batch, void (*)(imexlb_batch *)> b(new (std::nothrow) imexlb_batch,
+                                                              [](imexlb_batch *x) {
+                                                                  if (x) {
+                                                                      x->release();
+                                                                      delete x;
+                                                                  }
+                                                              });
+    if (!b) return fail(IMEXLB_E_OOM, "batch");
+    b->ncase = ncase;
+    b->prec = precision;
+    b->device = device;
+    b->ssz = precision == 0 ? 8 : precision == 1 ? 4 : 2;
+    std::vector<int64_t> fd, fs, lo, lbk;
+    std::vector<int8_t> lq;
+    std::vector<uint8_t> geo;
+    for (int i = 0; i < ncase; ++i) {
+        imexlb_ctx *c = nullptr;
+        int rc = imexlb_create(&c, 2, nxny[2 * i], nxny[2 * i + 1], 0, -1, precision);
+        if (rc) return rc;
+        b->host.push_back(c);
+        c->ssz = b->ssz;
+        const double *pr = params + 5 * i;  // pos, radius, ome, rho0, uu
+        if ((rc = imexlb_set_params(c, pr[2], pr[3], pr[4])) || (rc = imexlb_set_geometry(c, pr[0], pr[1])))
+            return rc;
+        DeviceTables t;
+        if ((rc = build_device_tables(c, t))) return rc;
+        BatchCase bc;
+        bc.kp = c->kp;
+        bc.pop_base = b->pop_total;
+        bc.geo_base = b->cell_total;
+        b->cases.push_back(bc);
+        const int64_t pb = bc.pop_base;
+        for (size_t k = 0; k < t.fill_dst.size(); ++k) {
+            const int64_t s = t.fill_src[k];
+            fd.push_back(t.fill_dst[k] + pb);
+            fs.push_back(s >= 0 ? s + pb : s == -1 ? -1 : encode_prev(decode_prev(s) + pb));
+        }
+        b->link_base.push_back(int64_t(lo.size()));
+        b->link_count.push_back(int64_t(t.link_out.size()));
+        b->max_links = std::max<int64_t>(b->max_links, int64_t(t.link_out.size()));
+        for (size_t k = 0; k < t.link_out.size(); ++k) {
+            lo.push_back(t.link_out[k] + pb);
+            lbk.push_back(t.link_back[k] + pb);
+            lq.push_back(t.link_q[k]);
+        }
+        geo.insert(geo.end(), t.geo.begin(), t.geo.end());
+        b->pop_total += int64_t(c->Q) * c->plane;
+        b->cell_total += c->plane;
+        b->node_total += int64_t(c->kp.N);
+    }
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    b->persist_grid = unsigned(std::max(cus, 1));
+    b->nfill = int64_t(fd.size());
+    const size_t pop = size_t(b->pop_total) * b->ssz, fld = size_t(b->cell_total) * sizeof(double);
+    const size_t nf = std::max<size_t>(fd.size(), 1), nl = std::max<size_t>(lo.size(), 1);
+    const size_t nblk = (size_t(std::max<int64_t>(b->max_links, 1)) + 255) / 256;
+    imexlb_batch *B = b.get();
+    int rc;
+    if ((rc = balloc(B, &B->fb[0], pop)) || (rc = balloc(B, &B->fb[1], pop)) || (rc = balloc(B, &B->f, pop)) ||
+        (rc = balloc(B, (void **)&B->p, fld)) || (rc = balloc(B, (void **)&B->ux, fld)) ||
+        (rc = balloc(B, (void **)&B->uy, fld)) || (rc = balloc(B, (void **)&B->rv, 100 * sizeof(double))) ||
+        (rc = balloc(B, (void **)&B->geo, size_t(B->cell_total))) ||
+        (rc = balloc(B, (void **)&B->fill_dst, nf * 8)) || (rc = balloc(B, (void **)&B->fill_src, nf * 8)) ||
+        (rc = balloc(B, (void **)&B->link_out, nl * 8)) || (rc = balloc(B, (void **)&B->link_back, nl * 8)) ||
+        (rc = balloc(B, (void **)&B->link_q, nl)) ||
+        (rc = balloc(B, (void **)&B->force_partial, nblk * 3 * sizeof(double))) ||
+        (rc = balloc(B, (void **)&B->force_out, size_t(3 * ncase) * sizeof(double))) ||
+        (rc = balloc(B, (void **)&B->d_cases, size_t(ncase) * sizeof(BatchCase))) ||
+        (rc = balloc(B, (void **)&B->queue, 64)))
+        return rc;
+    HIP_TRY(hipMemcpy(B->geo, geo.data(), geo.size(), hipMemcpyHostToDevice));
+    if (!fd.empty()) {
+        HIP_TRY(hipMemcpy(B->fill_dst, fd.data(), fd.size() * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(B->fill_src, fs.data(), fs.size() * 8, hipMemcpyHostToDevice));
+    }
+    if (!lo.empty()) {
+        HIP_TRY(hipMemcpy(B->link_out, lo.data(), lo.size() * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(B->link_back, lbk.data(), lbk.size() * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(B->link_q, lq.data(), lq.size(), hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipMemcpy(B->d_cases, B->cases.data(), size_t(ncase) * sizeof(BatchCase), hipMemcpyHostToDevice));
+    if ((rc = B->build_tiles())) return rc;
+    *out = b.release();
+    return IMEXLB_OK;
+}
+
+int imexlb_batch_destroy(imexlb_batch *b)
+{
+    if (!b) return IMEXLB_OK;
+    b->release();
+    delete b;
+    return IMEXLB_OK;
+}
+
+int imexlb_batch_init(imexlb_batch *b, const double *r_vec)
+{
+    CHECK_BATCH(b);
+    if (!r_vec) return fail(IMEXLB_E_ARG, "null r_vec");
+    HIP_TRY(hipMemcpyAsync(b->rv, r_vec, 100 * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    // both buffers start at zero: the body cells' previous values
+    HIP_TRY(hipMemsetAsync(b->fb[0], 0, size_t(b->pop_total) * b->ssz, b->stream));
+    HIP_TRY(hipMemsetAsync(b->fb[1], 0, size_t(b->pop_total) * b->ssz, b->stream));
+    for (int i = 0; i < b->ncase; ++i) {
+        StepArgs a = b->case_args(i);
+        hipError_t e = launch_init_up(2, a, b->rv, b->stream);
+        a.fout = b->pop(b->f, i);
+        if (e == hipSuccess) e = launch_init_pdf(2, b->prec, a, b->stream);
+        a.src = b->pop(b->f, i);
+        a.dst = b->pop(b->fb[1], i);
+        if (e == hipSuccess) e = launch_collide(2, b->prec, a, b->stream);
+        if (e != hipSuccess) return fail(IMEXLB_E_HIP, std::string("batch init: ") + hipGetErrorString(e));
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->cur = 1;
+    b->lazy = -1;
+    b->have_pu = true;
+    b->initialized = true;
+    return IMEXLB_OK;
+}
+
+int imexlb_batch_set_config(imexlb_batch *b, int block, int cells_per_thread, int persistent)
+{
+    CHECK_BATCH(b);
+    if ((block != 64 && block != 128 && block != 256 && block != 512) ||
+        (cells_per_thread != 1 && cells_per_thread != 2 && cells_per_thread != 4))
+        return fail(IMEXLB_E_ARG, "block in {64,128,256,512}, cells per thread in {1,2,4}");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->drop_graphs();
+    b->block = block;
+    b->cpt = cells_per_thread;
+    b->persist = persistent ? 1 : 0;
+    return b->build_tiles();
+}
+
+int imexlb_batch_advance(imexlb_batch *b, int64_t nsteps)
+{
+    CHECK_BATCH(b);
+    if (!b->initialized) return fail(IMEXLB_E_STATE, "imexlb_batch_init first");
+    if (nsteps < 0) return fail(IMEXLB_E_ARG, "negative step count");
+    int64_t pairs = nsteps / 2;
+    if (pairs > 0) {
+        hipGraphExec_t g;
+        int rc = b->pair_graph(&g);
+        if (rc) return rc;
+        for (int64_t k = 0; k < pairs; ++k) HIP_TRY(hipGraphLaunch(g, b->stream));
+        // a pair returns to the same buffers; its second step streamed from 1-cur
+        b->lazy = 1 - b->cur;
+        b->have_pu = false;
+    }
+    if (nsteps % 2) {
+        int rc = b->one_step();
+        if (rc) return rc;
+    }
+    return IMEXLB_OK;
+}
+
+int imexlb_batch_tune(imexlb_batch *b, int trial_steps, int32_t best[3], double *best_ms)
+{
+    CHECK_BATCH(b);
+    if (!b->initialized) return fail(IMEXLB_E_STATE, "imexlb_batch_init first");
+    if (trial_steps < 2) trial_steps = 2;
+    trial_steps += trial_steps % 2;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    double bms = 1e300;
+    int bb = b->block, bc = b->cpt, bp = b->persist;
+    for (int blk : {64, 128, 256, 512})
+        for (int cpt : {1, 2, 4})
+            for (int per : {0, 1}) {
+                int rc = imexlb_batch_set_config(b, blk, cpt, per);
+                if (!rc) rc = imexlb_batch_advance(b, 2);  // warm-up, builds the graph
+                if (rc) return rc;
+                HIP_TRY(hipEventRecord(e0, b->stream));
+                if ((rc = imexlb_batch_advance(b, trial_steps))) return rc;
+                HIP_TRY(hipEventRecord(e1, b->stream));
+                HIP_TRY(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                const double per_step = double(ms) / trial_steps;
+                if (per_step < bms) {
+                    bms = per_step;
+                    bb = blk;
+                    bc = cpt;
+                    bp = per;
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    int rc = imexlb_batch_set_config(b, bb, bc, bp);
+    if (rc) return rc;
+    if (best) {
+        best[0] = bb;
+        best[1] = bc;
+        best[2] = bp;
+    }
+    if (best_ms) *best_ms = bms;
+    return IMEXLB_OK;
+}
+
+int imexlb_batch_forces(imexlb_batch *b, double *F)
+{
+    CHECK_BATCH(b);
+    if (!F) return fail(IMEXLB_E_ARG, "null pointer");
+    if (b->lazy < 0) return fail(IMEXLB_E_STATE, "forces need a completed step");
+    for (int i = 0; i < b->ncase; ++i) {
+        const int64_t lb = b->link_base[size_t(i)];
+        hipError_t e = launch_forces(2, b->prec, b->fb[b->lazy], b->link_out + lb, b->link_back + lb,
+                                     b->link_q + lb, b->link_count[size_t(i)], b->force_partial,
+                                     b->force_out + 3 * i, b->stream);
+        if (e != hipSuccess) return fail(IMEXLB_E_HIP, std::string("batch forces: ") + hipGetErrorString(e));
+    }
+    std::vector<double> h(size_t(3 * b->ncase));
+    HIP_TRY(hipMemcpyAsync(h.data(), b->force_out, h.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    std::memcpy(F, h.data(), h.size() * sizeof(double));
+    return IMEXLB_OK;
+}
+
+int imexlb_batch_download_fields(imexlb_batch *b, int i, double *p, double *u)
+{
+    CHECK_BATCH(b);
+    if (i < 0 || i >= b->ncase || !p || !u) return fail(IMEXLB_E_ARG, "case index or null output");
+    if (!b->initialized) return fail(IMEXLB_E_STATE, "imexlb_batch_init first");
+    int rc = b->materialize(i);
+    if (rc) return rc;
+    const imexlb_ctx *c = b->host[size_t(i)];
+    const size_t n = size_t(c->plane);
+    std::vector<double> h(n);
+    const int64_t N = c->kp.N;
+    double *srcs[3] = {b->p, b->ux, b->uy};
+    for (int k = 0; k < 3; ++k) {
+        HIP_TRY(hipMemcpyAsync(h.data(), srcs[k] + b->cases[size_t(i)].geo_base, n * sizeof(double),
+                               hipMemcpyDeviceToHost, b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        int64_t m = 0;
+        for (int y = 0; y <= c->kp.ny; ++y)
+            for (int x = 0; x <= c->kp.nx; ++x, ++m) {
+                const double v = h[size_t(c->pidx(x, y, 0))];
+                if (k == 0)
+                    p[m] = v;
+                else
+                    u[int64_t(k - 1) * N + m] = v;
+            }
+    }
+    return IMEXLB_OK;
+}
+
+int imexlb_batch_info(imexlb_batch *b, int64_t out[4])
+{
+    if (!b || !out) return fail(IMEXLB_E_ARG, "null pointer");
+    out[0] = b->node_total;
+    out[1] = b->ncase;
+    out[2] = int64_t(b->ntiles);
+    out[3] = int64_t(b->dev_bytes);
+    return IMEXLB_OK;
+}
+
+}  // extern "C"

@@ -21,211 +21,12 @@
 #include <string>
 #include <vector>
 
-#include "../../include/imexlb.h"
-#include "geometry.hpp"
-#include "kernels.hpp"
+#include "context.hpp"
 
-using namespace imexlb;
+
 
 namespace {
 
-thread_local std::string g_last_error;
-
-int fail(int code, const std::string &msg)
-{
-    g_last_error = msg;
-    return code;
-}
-
-#define HIP_TRY(expr)                                                                                  \
-    do {                                                                                               \
-        hipError_t e_ = (expr);                                                                        \
-        if (e_ != hipSuccess)                                                                          \
-            return fail(e_ == hipErrorOutOfMemory ? IMEXLB_E_OOM : IMEXLB_E_HIP,                       \
-                        std::string(#expr) + ": " + hipGetErrorString(e_));                            \
-    } while (0)
-
-enum Stage { S_NONE, S_PRE, S_COLLIDED, S_FILLED, S_PENDING, S_STREAMED, S_FUSED };
-
-struct TimedLaunch {
-    int cls;
-    hipEvent_t a, b;
-};
-
-}  // namespace
-
-struct imexlb_ctx {
-    int dim = 2, Q = 9, device = -1, prec = 0;
-    bool host_only = true;
-    RefGeometry g;
-    BcTable bc;
-    bool geom_set = false;
-    double ome = 0.0, rho0 = 1.0, uu = 0.0;
-
-    // device layout
-    int64_t Px = 0, Py = 0, Pz = 1, PxPy = 0, o0 = 0, plane = 0;
-    size_t ssz = 8;
-    KParams kp{};
-    uint32_t skip_c = 0xffffffffu;
-
-    hipStream_t stream = nullptr;
-    void *fb[2] = {nullptr, nullptr};
-    void *f = nullptr;
-    double *p = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr, *rv = nullptr;
-    uint8_t *geo = nullptr;
-    int64_t *fill_dst = nullptr, *fill_src = nullptr;
-    int64_t nfill = 0;
-    unsigned long long *red = nullptr;
-    // body links for the momentum-exchange force
-    int64_t *link_out = nullptr, *link_back = nullptr;
-    int8_t *link_q = nullptr;
-    int64_t nlink = 0;
-    double *force_partial = nullptr, *force_out = nullptr;
-    size_t dev_bytes = 0;
-
-    // phase state
-    Stage stage = S_NONE;
-    int cur = 0;
-    bool fused = true;
-    bool have_pu = false, have_f = false, have_fb = false;
-    int lazy_src = -1;
-
-    // replay of two fused steps as a HIP graph, one per starting buffer
-    bool graphs = true;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    void drop_graphs()
-    {
-        for (auto &g : graph)
-            if (g) {
-                (void)hipGraphExecDestroy(g);
-                g = nullptr;
-            }
-    }
-
-    // profiling
-    bool prof = false;
-    std::vector<TimedLaunch> timed;
-    int64_t prof_n[3] = {0, 0, 0};
-    double prof_ms[3] = {0, 0, 0};
-
-    int64_t pidx(int x, int y, int z) const { return o0 + x + Px * y + PxPy * z; }
-
-    StepArgs args() const
-    {
-        StepArgs a;
-        a.kp = kp;
-        a.p = p;
-        a.ux = ux;
-        a.uy = uy;
-        a.uz = uz;
-        a.geo = geo;
-        return a;
-    }
-
-    int begin(int cls, hipEvent_t &b)
-    {
-        (void)cls;
-        if (!prof) return IMEXLB_OK;
-        HIP_TRY(hipEventCreate(&b));
-        HIP_TRY(hipEventRecord(b, stream));
-        return IMEXLB_OK;
-    }
-    int end(int cls, hipEvent_t b)
-    {
-        if (!prof) return IMEXLB_OK;
-        hipEvent_t e;
-        HIP_TRY(hipEventCreate(&e));
-        HIP_TRY(hipEventRecord(e, stream));
-        timed.push_back(TimedLaunch{cls, b, e});
-        return IMEXLB_OK;
-    }
-
-    template <class F>
-    int launch(int cls, F &&fn)
-    {
-        hipEvent_t b = nullptr;
-        int rc = begin(cls, b);
-        if (rc) return rc;
-        hipError_t e = fn();
-        if (e != hipSuccess) return fail(IMEXLB_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
-        return end(cls, b);
-    }
-
-    // BoundaryCondition on buffer `buf`; the other buffer holds the previous
-    // step's state (every Collision writes the buffer the previous step did
-    // not use)
-    int run_fill(int buf)
-    {
-        return launch(1, [&] { return launch_fill(prec, fb[buf], fb[1 - buf], fill_dst, fill_src, nfill, stream); });
-    }
-
-    int run_step(int mode, const void *src, void *dst, void *fout)
-    {
-        StepArgs a = args();
-        a.src = src;
-        a.dst = dst;
-        a.fout = fout;
-        return launch(mode == K_COLLIDE ? 0 : 2, [&] { return launch_step(dim, prec, mode, a, stream); });
-    }
-
-    // rebuild f, p, u of the current pre-collision state if the fused
-    // kernel skipped writing them
-    int materialize()
-    {
-        if (have_pu && have_f) return IMEXLB_OK;
-        if (lazy_src < 0) {
-            if (have_pu) return IMEXLB_OK;
-            return fail(IMEXLB_E_STATE, "fields requested before InitUP/upload");
-        }
-        int rc = run_step(K_WRITE_F | K_WRITE_PU, fb[lazy_src], nullptr, f);
-        if (rc) return rc;
-        have_pu = have_f = true;
-        return IMEXLB_OK;
-    }
-
-    void release()
-    {
-        if (host_only) return;
-        (void)hipSetDevice(device);
-        if (stream) (void)hipStreamSynchronize(stream);
-        drop_graphs();
-        for (auto &t : timed) {
-            (void)hipEventDestroy(t.a);
-            (void)hipEventDestroy(t.b);
-        }
-        timed.clear();
-        void *ptrs[] = {fb[0], fb[1], f,        p,         ux,     uy,           uz,       rv,
-                        geo,   fill_dst, fill_src, red, link_out, link_back, link_q, force_partial, force_out};
-        for (void *q : ptrs)
-            if (q) (void)hipFree(q);
-        fb[0] = fb[1] = f = nullptr;
-        p = ux = uy = uz = rv = nullptr;
-        geo = nullptr;
-        fill_dst = fill_src = nullptr;
-        red = nullptr;
-        link_out = link_back = nullptr;
-        link_q = nullptr;
-        force_partial = force_out = nullptr;
-        if (stream) (void)hipStreamDestroy(stream);
-        stream = nullptr;
-    }
-};
-
-namespace {
-
-int dev_alloc(imexlb_ctx *c, void **ptr, size_t bytes)
-{
-    hipError_t e = hipMalloc(ptr, bytes);
-    if (e != hipSuccess) {
-        *ptr = nullptr;
-        return fail(e == hipErrorOutOfMemory ? IMEXLB_E_OOM : IMEXLB_E_HIP,
-                    "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
-    }
-    c->dev_bytes += bytes;
-    e = hipMemset(*ptr, 0, bytes);
-    if (e != hipSuccess) return fail(IMEXLB_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
-    return IMEXLB_OK;
-}
 
 #define CHECK_CTX(c)                                                       \
     do {                                                                   \
@@ -298,6 +99,67 @@ double load_value(const uint8_t *base, size_t i, int prec)
 
 }  // namespace
 
+namespace imexlb {
+
+int build_device_tables(const imexlb_ctx *c, DeviceTables &t)
+{
+    const RefGeometry &g = c->g;
+    // node types: 1 on interior fluid nodes
+    t.geo.assign(size_t(c->plane), 0);
+    for (int32_t id : g.fluid_id) {
+        int x, y, z;
+        g.coords(id, x, y, z);
+        t.geo[size_t(c->pidx(x, y, z))] = 1;
+    }
+    if (!g.fluid_id.empty()) {
+        int x, y, z;
+        g.coords(g.fluid_id[0], x, y, z);
+        t.skip_c = uint32_t(c->pidx(x, y, z));
+    }
+    // resolved boundary table in device slots
+    auto dev_slot = [&](int64_t ref_slot) {
+        int x, y, z;
+        g.coords(ref_slot / c->Q, x, y, z);
+        return pop_slot(ref_slot % c->Q, c->pidx(x, y, z), c->plane, c->Q);
+    };
+    const int64_t nslots = int64_t(c->Q) * c->plane;
+    t.fill_dst.clear();
+    t.fill_src.clear();
+    t.fill_dst.reserve(c->bc.entries.size());
+    t.fill_src.reserve(c->bc.entries.size());
+    for (const FillEntry &e : c->bc.entries) {
+        const int64_t d = dev_slot(e.ref_slot);
+        int64_t s = -1;  // zero
+        if (e.ref_src >= 0) s = dev_slot(e.ref_src);
+        else if (e.ref_src <= -2) s = encode_prev(dev_slot(decode_prev(e.ref_src)));
+        const int64_t sabs = s >= 0 ? s : s == -1 ? 0 : decode_prev(s);
+        if (d < 0 || d >= nslots || sabs < 0 || sabs >= nslots)
+            return fail(IMEXLB_E_ARG, "boundary table slot outside the population arrays");
+        t.fill_dst.push_back(d);
+        t.fill_src.push_back(s);
+    }
+    // body links: fluid node x, direction q, interior solid neighbour x+e_q
+    t.link_out.clear();
+    t.link_back.clear();
+    t.link_q.clear();
+    const int nzmax = c->dim == 3 ? c->kp.nz : 0;
+    for (int z = 0; z <= nzmax; ++z)
+        for (int y = 0; y <= c->kp.ny; ++y)
+            for (int x = 0; x <= c->kp.nx; ++x) {
+                if (g.geo[size_t(g.id(x, y, z))] != 0) continue;
+                for (int q = 0; q < c->Q; ++q) {
+                    const int fx = x - g.lat.e[q][0], fy = y - g.lat.e[q][1], fz = z - g.lat.e[q][2];
+                    if (!g.interior(fx, fy, fz) || g.geo[size_t(g.id(fx, fy, fz))] != 1) continue;
+                    t.link_out.push_back(pop_slot(q, c->pidx(fx, fy, fz), c->plane, c->Q));
+                    t.link_back.push_back(pop_slot(c->Q - 1 - q, c->pidx(x, y, z), c->plane, c->Q));
+                    t.link_q.push_back(int8_t(q));
+                }
+            }
+    return IMEXLB_OK;
+}
+
+}  // namespace imexlb
+
 extern "C" {
 
 const char *imexlb_strerror(int code)
@@ -314,7 +176,7 @@ const char *imexlb_strerror(int code)
     }
 }
 
-const char *imexlb_last_error(void) { return g_last_error.c_str(); }
+const char *imexlb_last_error(void) { return last_error().c_str(); }
 
 int imexlb_device_count(int *n)
 {
@@ -445,92 +307,35 @@ int imexlb_set_geometry(imexlb_ctx *c, double pos, double radius)
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->drop_graphs();
-    // device mask: 1 on interior fluid nodes
-    std::vector<uint8_t> geo(size_t(c->plane), 0);
-    const RefGeometry &g = c->g;
-    for (int32_t id : g.fluid_id) {
-        int x, y, z;
-        g.coords(id, x, y, z);
-        geo[size_t(c->pidx(x, y, z))] = 1;
-    }
-    if (!g.fluid_id.empty()) {
-        int x, y, z;
-        g.coords(g.fluid_id[0], x, y, z);
-        c->skip_c = uint32_t(c->pidx(x, y, z));
-    }
-    HIP_TRY(hipMemcpy(c->geo, geo.data(), geo.size(), hipMemcpyHostToDevice));
-
-    // resolved boundary table in device slots
-    std::vector<int64_t> dst, src;
-    dst.reserve(c->bc.entries.size());
-    src.reserve(c->bc.entries.size());
-    auto dev_slot = [&](int64_t ref_slot) {
-        int x, y, z;
-        g.coords(ref_slot / c->Q, x, y, z);
-        return pop_slot(ref_slot % c->Q, c->pidx(x, y, z), c->plane, c->Q);
-    };
-    const int64_t nslots = int64_t(c->Q) * c->plane;
-    for (const FillEntry &e : c->bc.entries) {
-        const int64_t d = dev_slot(e.ref_slot);
-        int64_t s = -1;  // zero
-        if (e.ref_src >= 0) s = dev_slot(e.ref_src);
-        else if (e.ref_src <= -2) s = encode_prev(dev_slot(decode_prev(e.ref_src)));
-        const int64_t sabs = s >= 0 ? s : s == -1 ? 0 : decode_prev(s);
-        if (d < 0 || d >= nslots || sabs < 0 || sabs >= nslots)
-            return fail(IMEXLB_E_ARG, "boundary table slot outside the population arrays");
-        dst.push_back(d);
-        src.push_back(s);
-    }
-    if (c->fill_dst) (void)hipFree(c->fill_dst);
-    if (c->fill_src) (void)hipFree(c->fill_src);
-    c->fill_dst = c->fill_src = nullptr;
-    c->nfill = int64_t(dst.size());
-    if (c->nfill) {
-        int rc;
-        if ((rc = dev_alloc(c, (void **)&c->fill_dst, dst.size() * 8)) ||
-            (rc = dev_alloc(c, (void **)&c->fill_src, src.size() * 8)))
-            return rc;
-        HIP_TRY(hipMemcpy(c->fill_dst, dst.data(), dst.size() * 8, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(c->fill_src, src.data(), src.size() * 8, hipMemcpyHostToDevice));
-    }
-
-    // body links: fluid node x, direction q, interior solid neighbour x+e_q
-    std::vector<int64_t> lo, lb;
-    std::vector<int8_t> lq;
-    const int nzmax = c->dim == 3 ? c->kp.nz : 0;
-    for (int z = 0; z <= nzmax; ++z)
-        for (int y = 0; y <= c->kp.ny; ++y)
-            for (int x = 0; x <= c->kp.nx; ++x) {
-                if (g.geo[size_t(g.id(x, y, z))] != 0) continue;
-                for (int q = 0; q < c->Q; ++q) {
-                    const int fx = x - g.lat.e[q][0], fy = y - g.lat.e[q][1], fz = z - g.lat.e[q][2];
-                    if (!g.interior(fx, fy, fz) || g.geo[size_t(g.id(fx, fy, fz))] != 1) continue;
-                    lo.push_back(pop_slot(q, c->pidx(fx, fy, fz), c->plane, c->Q));
-                    lb.push_back(pop_slot(c->Q - 1 - q, c->pidx(x, y, z), c->plane, c->Q));
-                    lq.push_back(int8_t(q));
-                }
-            }
-    for (void *q : {(void *)c->link_out, (void *)c->link_back, (void *)c->link_q, (void *)c->force_partial,
-                    (void *)c->force_out})
+    DeviceTables t;
+    int rc = build_device_tables(c, t);
+    if (rc) return rc;
+    c->skip_c = t.skip_c;
+    HIP_TRY(hipMemcpy(c->geo, t.geo.data(), t.geo.size(), hipMemcpyHostToDevice));
+    for (void *q : {(void *)c->fill_dst, (void *)c->fill_src, (void *)c->link_out, (void *)c->link_back,
+                    (void *)c->link_q, (void *)c->force_partial, (void *)c->force_out})
         if (q) (void)hipFree(q);
-    c->link_out = c->link_back = nullptr;
+    c->fill_dst = c->fill_src = c->link_out = c->link_back = nullptr;
     c->link_q = nullptr;
     c->force_partial = c->force_out = nullptr;
-    c->nlink = int64_t(lo.size());
-    {
-        const size_t nl = std::max<size_t>(lo.size(), 1);
-        const size_t nblk = (nl + 255) / 256;
-        int rc;
-        if ((rc = dev_alloc(c, (void **)&c->link_out, nl * 8)) || (rc = dev_alloc(c, (void **)&c->link_back, nl * 8)) ||
-            (rc = dev_alloc(c, (void **)&c->link_q, nl)) ||
-            (rc = dev_alloc(c, (void **)&c->force_partial, nblk * 3 * sizeof(double))) ||
-            (rc = dev_alloc(c, (void **)&c->force_out, 4 * sizeof(double))))
-            return rc;
-        if (!lo.empty()) {
-            HIP_TRY(hipMemcpy(c->link_out, lo.data(), lo.size() * 8, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(c->link_back, lb.data(), lb.size() * 8, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(c->link_q, lq.data(), lq.size(), hipMemcpyHostToDevice));
-        }
+    c->nfill = int64_t(t.fill_dst.size());
+    c->nlink = int64_t(t.link_out.size());
+    const size_t nf = std::max<size_t>(t.fill_dst.size(), 1), nl = std::max<size_t>(t.link_out.size(), 1);
+    const size_t nblk = (nl + 255) / 256;
+    if ((rc = dev_alloc(c, (void **)&c->fill_dst, nf * 8)) || (rc = dev_alloc(c, (void **)&c->fill_src, nf * 8)) ||
+        (rc = dev_alloc(c, (void **)&c->link_out, nl * 8)) || (rc = dev_alloc(c, (void **)&c->link_back, nl * 8)) ||
+        (rc = dev_alloc(c, (void **)&c->link_q, nl)) ||
+        (rc = dev_alloc(c, (void **)&c->force_partial, nblk * 3 * sizeof(double))) ||
+        (rc = dev_alloc(c, (void **)&c->force_out, 4 * sizeof(double))))
+        return rc;
+    if (c->nfill) {
+        HIP_TRY(hipMemcpy(c->fill_dst, t.fill_dst.data(), t.fill_dst.size() * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->fill_src, t.fill_src.data(), t.fill_src.size() * 8, hipMemcpyHostToDevice));
+    }
+    if (c->nlink) {
+        HIP_TRY(hipMemcpy(c->link_out, t.link_out.data(), t.link_out.size() * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->link_back, t.link_back.data(), t.link_back.size() * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->link_q, t.link_q.data(), t.link_q.size(), hipMemcpyHostToDevice));
     }
     c->stage = S_NONE;
     c->have_pu = c->have_f = c->have_fb = false;

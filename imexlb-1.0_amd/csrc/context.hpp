@@ -1,0 +1,233 @@
+// context.hpp -- internal: the lattice context behind the C ABI, shared by
+// capi.cpp (single lattice) and batch.cpp (batched ensemble).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/imexlb.h"
+#include "geometry.hpp"
+#include "kernels.hpp"
+
+namespace imexlb {
+
+inline std::string &last_error()
+{
+    thread_local std::string s;
+    return s;
+}
+
+inline int fail(int code, const std::string &msg)
+{
+    last_error() = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(e_ == hipErrorOutOfMemory ? IMEXLB_E_OOM : IMEXLB_E_HIP,                       \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                            \
+    } while (0)
+
+enum Stage { S_NONE, S_PRE, S_COLLIDED, S_FILLED, S_PENDING, S_STREAMED, S_FUSED };
+
+struct TimedLaunch {
+    int cls;
+    hipEvent_t a, b;
+};
+
+
+}  // namespace imexlb
+
+using namespace imexlb;
+
+struct imexlb_ctx {
+    int dim = 2, Q = 9, device = -1, prec = 0;
+    bool host_only = true;
+    RefGeometry g;
+    BcTable bc;
+    bool geom_set = false;
+    double ome = 0.0, rho0 = 1.0, uu = 0.0;
+
+    // device layout
+    int64_t Px = 0, Py = 0, Pz = 1, PxPy = 0, o0 = 0, plane = 0;
+    size_t ssz = 8;
+    KParams kp{};
+    uint32_t skip_c = 0xffffffffu;
+
+    hipStream_t stream = nullptr;
+    void *fb[2] = {nullptr, nullptr};
+    void *f = nullptr;
+    double *p = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr, *rv = nullptr;
+    uint8_t *geo = nullptr;
+    int64_t *fill_dst = nullptr, *fill_src = nullptr;
+    int64_t nfill = 0;
+    unsigned long long *red = nullptr;
+    // body links for the momentum-exchange force
+    int64_t *link_out = nullptr, *link_back = nullptr;
+    int8_t *link_q = nullptr;
+    int64_t nlink = 0;
+    double *force_partial = nullptr, *force_out = nullptr;
+    size_t dev_bytes = 0;
+
+    // phase state
+    Stage stage = S_NONE;
+    int cur = 0;
+    bool fused = true;
+    bool have_pu = false, have_f = false, have_fb = false;
+    int lazy_src = -1;
+
+    // replay of two fused steps as a HIP graph, one per starting buffer
+    bool graphs = true;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    void drop_graphs()
+    {
+        for (auto &g : graph)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+    }
+
+    // profiling
+    bool prof = false;
+    std::vector<TimedLaunch> timed;
+    int64_t prof_n[3] = {0, 0, 0};
+    double prof_ms[3] = {0, 0, 0};
+
+    int64_t pidx(int x, int y, int z) const { return o0 + x + Px * y + PxPy * z; }
+
+    StepArgs args() const
+    {
+        StepArgs a;
+        a.kp = kp;
+        a.p = p;
+        a.ux = ux;
+        a.uy = uy;
+        a.uz = uz;
+        a.geo = geo;
+        return a;
+    }
+
+    int begin(int cls, hipEvent_t &b)
+    {
+        (void)cls;
+        if (!prof) return IMEXLB_OK;
+        HIP_TRY(hipEventCreate(&b));
+        HIP_TRY(hipEventRecord(b, stream));
+        return IMEXLB_OK;
+    }
+    int end(int cls, hipEvent_t b)
+    {
+        if (!prof) return IMEXLB_OK;
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        HIP_TRY(hipEventRecord(e, stream));
+        timed.push_back(TimedLaunch{cls, b, e});
+        return IMEXLB_OK;
+    }
+
+    template <class F>
+    int launch(int cls, F &&fn)
+    {
+        hipEvent_t b = nullptr;
+        int rc = begin(cls, b);
+        if (rc) return rc;
+        hipError_t e = fn();
+        if (e != hipSuccess) return fail(IMEXLB_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+        return end(cls, b);
+    }
+
+    // BoundaryCondition on buffer `buf`; the other buffer holds the previous
+    // step's state (every Collision writes the buffer the previous step did
+    // not use)
+    int run_fill(int buf)
+    {
+        return launch(1, [&] { return launch_fill(prec, fb[buf], fb[1 - buf], fill_dst, fill_src, nfill, stream); });
+    }
+
+    int run_step(int mode, const void *src, void *dst, void *fout)
+    {
+        StepArgs a = args();
+        a.src = src;
+        a.dst = dst;
+        a.fout = fout;
+        return launch(mode == K_COLLIDE ? 0 : 2, [&] { return launch_step(dim, prec, mode, a, stream); });
+    }
+
+    // rebuild f, p, u of the current pre-collision state if the fused
+    // kernel skipped writing them
+    int materialize()
+    {
+        if (have_pu && have_f) return IMEXLB_OK;
+        if (lazy_src < 0) {
+            if (have_pu) return IMEXLB_OK;
+            return fail(IMEXLB_E_STATE, "fields requested before InitUP/upload");
+        }
+        int rc = run_step(K_WRITE_F | K_WRITE_PU, fb[lazy_src], nullptr, f);
+        if (rc) return rc;
+        have_pu = have_f = true;
+        return IMEXLB_OK;
+    }
+
+    void release()
+    {
+        if (host_only) return;
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graphs();
+        for (auto &t : timed) {
+            (void)hipEventDestroy(t.a);
+            (void)hipEventDestroy(t.b);
+        }
+        timed.clear();
+        void *ptrs[] = {fb[0], fb[1], f,        p,         ux,     uy,           uz,       rv,
+                        geo,   fill_dst, fill_src, red, link_out, link_back, link_q, force_partial, force_out};
+        for (void *q : ptrs)
+            if (q) (void)hipFree(q);
+        fb[0] = fb[1] = f = nullptr;
+        p = ux = uy = uz = rv = nullptr;
+        geo = nullptr;
+        fill_dst = fill_src = nullptr;
+        red = nullptr;
+        link_out = link_back = nullptr;
+        link_q = nullptr;
+        force_partial = force_out = nullptr;
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+};
+
+namespace imexlb {
+
+inline int dev_alloc(imexlb_ctx *c, void **ptr, size_t bytes)
+{
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return fail(e == hipErrorOutOfMemory ? IMEXLB_E_OOM : IMEXLB_E_HIP,
+                    "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    c->dev_bytes += bytes;
+    e = hipMemset(*ptr, 0, bytes);
+    if (e != hipSuccess) return fail(IMEXLB_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    return IMEXLB_OK;
+}
+
+// Device-layout tables of one lattice, built on the host from its geometry:
+// node types, the resolved boundary fill (dst, src in FillEntry encoding
+// with device slots) and the body links of the force.
+struct DeviceTables {
+    std::vector<uint8_t> geo;
+    uint32_t skip_c = 0xffffffffu;
+    std::vector<int64_t> fill_dst, fill_src;
+    std::vector<int64_t> link_out, link_back;
+    std::vector<int8_t> link_q;
+};
+int build_device_tables(const imexlb_ctx *c, DeviceTables &t);
+
+}  // namespace imexlb

@@ -62,6 +62,14 @@ inline int64_t pop_slot(int64_t q, int64_t c, int64_t plane, int Q)
 
 enum : int { K_COLLIDE = 1, K_WRITE_F = 2, K_WRITE_PU = 4, K_NOSHIFT = 8 };
 
+// One case of a batched ensemble: its parameters and where its populations
+// and node types start in the shared arrays.
+struct BatchCase {
+    KParams kp;
+    int64_t pop_base;  // first population slot of the case
+    int64_t geo_base;  // first cell of the case (node types, p, u)
+};
+
 struct StepArgs {
     KParams kp;
     const void *src = nullptr;  // population input
@@ -79,6 +87,14 @@ hipError_t launch_init_up(int dim, const StepArgs &a, const double *rv, hipStrea
 hipError_t launch_fill(int prec, void *buf, const void *prev, const int64_t *dslot, const int64_t *sslot, int64_t n,
                        hipStream_t s);
 hipError_t launch_monitor(int dim, const StepArgs &a, uint32_t skip, unsigned long long *out, hipStream_t s);
+// Batched ensemble (D2Q9; prec 0, 1 or 2).  block in {64,128,256,512},
+// cpt in {1,2,4}; persist launches `grid` blocks that take tiles from
+// `queue` (rewound by launch_batch_fill), otherwise one block per tile.
+hipError_t launch_batch_step(int prec, int block, int cpt, int persist, unsigned grid, const void *src, void *dst,
+                             const uint8_t *geo, const BatchCase *cases, const uint32_t *tile_case,
+                             const uint32_t *tile_first, uint32_t ntiles, uint32_t *queue, hipStream_t s);
+hipError_t launch_batch_fill(int prec, void *buf, const void *prev, const int64_t *dslot, const int64_t *sslot,
+                             int64_t n, uint32_t *queue, hipStream_t s);
 // body-link momentum exchange; partial holds 3 doubles per 256 links (at least 3)
 hipError_t launch_forces(int dim, int prec, const void *buf, const int64_t *out_slot, const int64_t *back_slot,
                          const int8_t *lq, int64_t n, double *partial, double *out, hipStream_t s);

@@ -286,16 +286,13 @@ __device__ __forceinline__ void collide(const C (&f)[Lat<D>::Q], C p, C ux, C uy
 // WRITE_F alone is Propagation; NOSHIFT|WRITE_PU is PostProcessing;
 // WRITE_F|WRITE_PU materialises f, p, u lazily.
 template <int D, class S, class C, int MODE>
-__global__ __launch_bounds__(IMEXLB_STEP_BLOCK) void k_step(const S *__restrict__ src, S *__restrict__ dst,
-                                              S *__restrict__ fout, double *__restrict__ pout,
-                                              double *__restrict__ uxo, double *__restrict__ uyo,
-                                              double *__restrict__ uzo, const uint8_t *__restrict__ geo,
-                                              KParams kp)
+__device__ __forceinline__ void step_node(const S *__restrict__ src, S *__restrict__ dst, S *__restrict__ fout,
+                                          double *__restrict__ pout, double *__restrict__ uxo,
+                                          double *__restrict__ uyo, double *__restrict__ uzo,
+                                          const uint8_t *__restrict__ geo, const KParams &kp, int x, int y, int z,
+                                          uint32_t c)
 {
     using L = Lat<D>;
-    int x, y, z;
-    uint32_t c;
-    if (!node_of<D>(kp, xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, x, y, z, c)) return;
     // The node's type is loaded together with its populations, not before
     // them, so the gather does not wait for it (every gathered slot is in
     // the arrays also for solid nodes).
@@ -340,6 +337,73 @@ __global__ __launch_bounds__(IMEXLB_STEP_BLOCK) void k_step(const S *__restrict_
                 *o = Store<S>::template st<C>(v);
         });
     }
+}
+
+template <int D, class S, class C, int MODE>
+__global__ __launch_bounds__(IMEXLB_STEP_BLOCK) void k_step(const S *__restrict__ src, S *__restrict__ dst,
+                                              S *__restrict__ fout, double *__restrict__ pout,
+                                              double *__restrict__ uxo, double *__restrict__ uyo,
+                                              double *__restrict__ uzo, const uint8_t *__restrict__ geo,
+                                              KParams kp)
+{
+    int x, y, z;
+    uint32_t c;
+    if (!node_of<D>(kp, xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, x, y, z, c)) return;
+    step_node<D, S, C, MODE>(src, dst, fout, pout, uxo, uyo, uzo, geo, kp, x, y, z, c);
+}
+
+// ---------------------------------------------------------------------------
+// Batched ensemble: many independent lattices advanced by one launch.  The
+// interior nodes of every case are cut into tiles of BLOCK*CPT nodes; a
+// block takes a tile (by its index, or from a queue when PERSIST), reads the
+// tile's case and runs the fused pull on CPT nodes per thread, strided by
+// BLOCK.  Every case keeps its own parameters, populations and node types.
+template <int D, class S, class C, int BLOCK, int CPT, int PERSIST>
+__global__ __launch_bounds__(BLOCK) void k_batch_step(const S *__restrict__ src, S *__restrict__ dst,
+                                                      const uint8_t *__restrict__ geo,
+                                                      const BatchCase *__restrict__ cases,
+                                                      const uint32_t *__restrict__ tile_case,
+                                                      const uint32_t *__restrict__ tile_first, uint32_t ntiles,
+                                                      uint32_t *__restrict__ queue)
+{
+    __shared__ uint32_t next;
+    uint32_t tile = PERSIST ? 0u : xcd_block(blockIdx.x, gridDim.x);
+    for (;;) {
+        if (PERSIST) {
+            __syncthreads();  // every thread has read the previous tile index
+            if (threadIdx.x == 0) next = atomicAdd(queue, 1u);
+            __syncthreads();
+            tile = next;
+        }
+        if (tile >= ntiles) return;  // every block of the queue ends here
+        const BatchCase &bc = cases[tile_case[tile]];
+        const uint32_t first = tile_first[tile];
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const uint32_t i = first + uint32_t(k * BLOCK) + threadIdx.x;
+            int x, y, z;
+            uint32_t c;
+            if (decode<D>(bc.kp, i, x, y, z, c))
+                step_node<D, S, C, K_COLLIDE>(src + bc.pop_base, dst + bc.pop_base, nullptr, nullptr, nullptr, nullptr,
+                                              nullptr, geo + bc.geo_base, bc.kp, x, y, z, c);
+        }
+        if (!PERSIST) return;
+    }
+}
+
+// Boundary fill of all cases (slots already offset per case); thread 0 also
+// rewinds the tile queue of the step kernel that follows on the stream.
+template <class S>
+__global__ __launch_bounds__(256) void k_batch_fill(S *__restrict__ buf, const S *__restrict__ prev,
+                                                    const int64_t *__restrict__ dslot,
+                                                    const int64_t *__restrict__ sslot, int64_t n,
+                                                    uint32_t *__restrict__ queue)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i == 0) *queue = 0u;
+    if (i >= n) return;
+    const int64_t s = sslot[i];
+    buf[dslot[i]] = s >= 0 ? buf[s] : s == -1 ? Store<S>::template st<float>(0.0f) : prev[-s - 2];
 }
 
 // Collision from stored f, p, u (lbm.f90:56-74): the first step after
@@ -641,6 +705,88 @@ hipError_t launch_monitor(int dim, const StepArgs &a, uint32_t skip, unsigned lo
         hipLaunchKernelGGL((k_monitor<3>), dim3(nblocks(a.kp.N)), dim3(256), 0, s, a.ux, a.uy, a.geo, skip, a.kp,
                            out);
     return hipGetLastError();
+}
+
+}  // namespace imexlb
+
+namespace imexlb {
+
+template <class S, class C, int BLOCK, int CPT, int PERSIST>
+static void batch_go(unsigned grid, const void *src, void *dst, const uint8_t *geo, const BatchCase *cases,
+                     const uint32_t *tc, const uint32_t *tf, uint32_t ntiles, uint32_t *queue, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_batch_step<2, S, C, BLOCK, CPT, PERSIST>), dim3(grid), dim3(BLOCK), 0, s, (const S *)src,
+                       (S *)dst, geo, cases, tc, tf, ntiles, queue);
+}
+
+template <class S, class C, int BLOCK, int CPT>
+static hipError_t batch_p(int persist, unsigned grid, const void *src, void *dst, const uint8_t *geo,
+                          const BatchCase *cases, const uint32_t *tc, const uint32_t *tf, uint32_t ntiles,
+                          uint32_t *queue, hipStream_t s)
+{
+    if (persist)
+        batch_go<S, C, BLOCK, CPT, 1>(grid, src, dst, geo, cases, tc, tf, ntiles, queue, s);
+    else
+        batch_go<S, C, BLOCK, CPT, 0>(ntiles, src, dst, geo, cases, tc, tf, ntiles, queue, s);
+    return hipGetLastError();
+}
+
+template <class S, class C, int BLOCK>
+static hipError_t batch_c(int cpt, int persist, unsigned grid, const void *src, void *dst, const uint8_t *geo,
+                          const BatchCase *cases, const uint32_t *tc, const uint32_t *tf, uint32_t ntiles,
+                          uint32_t *queue, hipStream_t s)
+{
+    switch (cpt) {
+    case 1: return batch_p<S, C, BLOCK, 1>(persist, grid, src, dst, geo, cases, tc, tf, ntiles, queue, s);
+    case 2: return batch_p<S, C, BLOCK, 2>(persist, grid, src, dst, geo, cases, tc, tf, ntiles, queue, s);
+    case 4: return batch_p<S, C, BLOCK, 4>(persist, grid, src, dst, geo, cases, tc, tf, ntiles, queue, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <class S, class C>
+static hipError_t batch_b(int block, int cpt, int persist, unsigned grid, const void *src, void *dst,
+                          const uint8_t *geo, const BatchCase *cases, const uint32_t *tc, const uint32_t *tf,
+                          uint32_t ntiles, uint32_t *queue, hipStream_t s)
+{
+    switch (block) {
+    case 64: return batch_c<S, C, 64>(cpt, persist, grid, src, dst, geo, cases, tc, tf, ntiles, queue, s);
+    case 128: return batch_c<S, C, 128>(cpt, persist, grid, src, dst, geo, cases, tc, tf, ntiles, queue, s);
+    case 256: return batch_c<S, C, 256>(cpt, persist, grid, src, dst, geo, cases, tc, tf, ntiles, queue, s);
+    case 512: return batch_c<S, C, 512>(cpt, persist, grid, src, dst, geo, cases, tc, tf, ntiles, queue, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_batch_step(int prec, int block, int cpt, int persist, unsigned grid, const void *src, void *dst,
+                             const uint8_t *geo, const BatchCase *cases, const uint32_t *tile_case,
+                             const uint32_t *tile_first, uint32_t ntiles, uint32_t *queue, hipStream_t s)
+{
+    if (ntiles == 0) return hipSuccess;
+    switch (prec) {
+    case 0:
+        return batch_b<double, double>(block, cpt, persist, grid, src, dst, geo, cases, tile_case, tile_first, ntiles,
+                                       queue, s);
+    case 1:
+        return batch_b<float, float>(block, cpt, persist, grid, src, dst, geo, cases, tile_case, tile_first, ntiles,
+                                     queue, s);
+    case 2:
+        return batch_b<_Float16, float>(block, cpt, persist, grid, src, dst, geo, cases, tile_case, tile_first, ntiles,
+                                        queue, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_batch_fill(int prec, void *buf, const void *prev, const int64_t *dslot, const int64_t *sslot,
+                             int64_t n, uint32_t *queue, hipStream_t s)
+{
+    const unsigned nb = n > 0 ? nblocks(uint64_t(n)) : 1u;
+    return dispatch_prec(prec, [&](auto *sp, auto *) {
+        using S = std::remove_pointer_t<decltype(sp)>;
+        hipLaunchKernelGGL((k_batch_fill<S>), dim3(nb), dim3(256), 0, s, (S *)buf, (const S *)prev, dslot, sslot, n,
+                           queue);
+        return hipGetLastError();
+    });
 }
 
 }  // namespace imexlb

@@ -53,6 +53,15 @@ SIGNATURES = {
     "imexlb_profile": (C.c_int, [_P, C.c_int]),
     "imexlb_profile_read": (C.c_int, [_P, _I64P, _DP]),
     "imexlb_info": (C.c_int, [_P, _I64P]),
+    "imexlb_batch_create": (C.c_int, [C.POINTER(_P), C.c_int, _I32P, _DP, C.c_int, C.c_int]),
+    "imexlb_batch_destroy": (C.c_int, [_P]),
+    "imexlb_batch_init": (C.c_int, [_P, _DP]),
+    "imexlb_batch_advance": (C.c_int, [_P, C.c_int64]),
+    "imexlb_batch_set_config": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
+    "imexlb_batch_tune": (C.c_int, [_P, C.c_int, _I32P, _DP]),
+    "imexlb_batch_forces": (C.c_int, [_P, _DP]),
+    "imexlb_batch_download_fields": (C.c_int, [_P, C.c_int, _DP, _DP]),
+    "imexlb_batch_info": (C.c_int, [_P, _I64P]),
 }
 
 

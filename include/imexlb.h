@@ -171,6 +171,39 @@ int imexlb_profile_read(imexlb_ctx *ctx, int64_t launches[3], double ms[3]);
  * population value, [5] boundary-table entries. */
 int imexlb_info(imexlb_ctx *ctx, int64_t out[6]);
 
+/* ---------------------------------------------------------------------
+ * Batched ensemble.  No counterpart in the reference, which runs one
+ * lattice per process: ncase independent D2Q9 lattices, each with its own
+ * size and parameters, advanced together by one boundary-fill and one step
+ * launch per time step (pairs of steps replayed as a HIP graph).  Every case
+ * follows the single-lattice semantics above exactly.
+ * --------------------------------------------------------------------- */
+typedef struct imexlb_batch imexlb_batch;
+
+/* nxny[2*i..2*i+1] = (nx, ny) of case i; params[5*i..5*i+4] = (pos, radius,
+ * ome, rho0, uu) as SetGeometry / DataInput define them.  precision:
+ * IMEXLB_F64, IMEXLB_F32 or IMEXLB_F16_F32. */
+int imexlb_batch_create(imexlb_batch **b, int ncase, const int32_t *nxny, const double *params, int device,
+                        int precision);
+int imexlb_batch_destroy(imexlb_batch *b);
+/* InitUP + InitPDF + the first Collision of every case (same r_vec). */
+int imexlb_batch_init(imexlb_batch *b, const double *r_vec);
+/* nsteps complete iterations of every case. */
+int imexlb_batch_advance(imexlb_batch *b, int64_t nsteps);
+/* Step-kernel shape: threads per block (64, 128, 256, 512), nodes per
+ * thread (1, 2, 4), persistent blocks taking tiles from a queue (1) or one
+ * block per tile (0).  Results do not depend on it. */
+int imexlb_batch_set_config(imexlb_batch *b, int block, int cells_per_thread, int persistent);
+/* Times every shape over trial_steps steps (advancing the ensemble) and
+ * keeps the fastest; best = (block, cells_per_thread, persistent). */
+int imexlb_batch_tune(imexlb_batch *b, int trial_steps, int32_t best[3], double *best_ms_per_step);
+/* Momentum-exchange force of every case, F[3*i..3*i+2]. */
+int imexlb_batch_forces(imexlb_batch *b, double *F);
+/* p and u of case i, interior layout (u[d][y][x]). */
+int imexlb_batch_download_fields(imexlb_batch *b, int i, double *p, double *u);
+/* [0] nodes of all cases, [1] cases, [2] tiles, [3] device bytes */
+int imexlb_batch_info(imexlb_batch *b, int64_t out[4]);
+
 #ifdef __cplusplus
 }
 #endif

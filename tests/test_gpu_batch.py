@@ -1,0 +1,78 @@
+"""Batched ensemble: every case of a batch evolves exactly like the same
+deck run as a single lattice (bit for bit in FP64, fields and forces), for
+every step-kernel shape the autotuner can pick."""
+import numpy as np
+import pytest
+
+import imexlb as I
+from imexlb.ensemble import Batch, cylinder_cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_cases():
+    decks = cylinder_cases(5, seed=11)
+    # shrink to test sizes, keeping each case's Re, relaxation and body ratio
+    out = []
+    for k, dk in enumerate(decks):
+        nx = 48 + 16 * k
+        ny = nx // 4
+        out.append(I.Deck(2, (nx, ny, 0), dk.Lc, dk.Xpos, dk.Re, dk.ome, 1.0, 1.0, 1.0, 0.01))
+    return out
+
+
+@pytest.mark.parametrize("block,cpt,persist", [(64, 1, 0), (256, 2, 1), (512, 4, 0), (128, 4, 1), (256, 1, 0)])
+def test_batch_matches_single_lattices(block, cpt, persist, r_vec):
+    decks = _small_cases()
+    print(f"[batch] block={block} cpt={cpt} persist={persist}", flush=True)
+    b = Batch(decks, precision=I.F64)
+    b.set_config(block, cpt, persist)
+    b.init(r_vec)
+    b.advance(37)
+    Fb = b.forces()
+    for i, dk in enumerate(decks):
+        lat, _ = I.setup(dk, r_vec, device=0)
+        lat.advance(37)
+        p, u = lat.download_fields()
+        pb, ub = b.download_fields(i)
+        assert np.array_equal(p, pb) and np.array_equal(u, ub), i
+        assert np.array_equal(lat.forces(), Fb[i]), i
+        lat.close()
+    b.close()
+
+
+def test_batch_tune_and_continue(r_vec):
+    """Tuning picks a valid shape; advancing after it (odd and even step
+    counts, through the graph and the single-step path) stays exact."""
+    decks = _small_cases()
+    b = Batch(decks, precision=I.F64)
+    b.init(r_vec)
+    best = b.tune(4)
+    assert best["block"] in (64, 128, 256, 512) and best["cells_per_thread"] in (1, 2, 4)
+    assert best["ms_per_step"] > 0
+    b.init(r_vec)
+    for n in (3, 8, 1):
+        b.advance(n)
+    lat, _ = I.setup(decks[2], r_vec, device=0)
+    lat.advance(12)
+    p, u = lat.download_fields()
+    pb, ub = b.download_fields(2)
+    assert np.array_equal(p, pb) and np.array_equal(u, ub)
+    lat.close()
+    b.close()
+
+
+def test_batch_fp16_close_to_fp64(r_vec):
+    decks = _small_cases()
+    b64 = Batch(decks, precision=I.F64)
+    b16 = Batch(decks, precision=I.F16_F32)
+    for b in (b64, b16):
+        b.init(r_vec)
+        b.advance(200)
+    for i in range(len(decks)):
+        _, u64 = b64.download_fields(i)
+        _, u16 = b16.download_fields(i)
+        err = np.linalg.norm(u16 - u64) / np.linalg.norm(u64)
+        assert err < 1e-2, (i, err)
+    b64.close()
+    b16.close()

@@ -9,6 +9,12 @@ the timed region.  The CPU baseline is the oracle (the C restatement of the
 reference) run on the host cores over a few steps of the same lattice.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-variants]
+    python bench.py --copy-calibration
+
+``roofline.traffic`` (HBM bytes per launch of the step kernel) is read from
+the counter summary in profiles/: FETCH_SIZE and WRITE_SIZE of the step
+kernel, each scaled by what the counter reads for a plain 256 MiB device copy
+(``--copy-calibration`` runs that copy).
 """
 import argparse
 import json
@@ -25,6 +31,8 @@ import imexlb as I  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 DECK = "sphere512x256x256"
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_step_sphere512x256x256_f64_hbm_traffic.md")
+CAL_BYTES = 256 * 1024 * 1024
 
 
 def log(msg):
@@ -66,6 +74,32 @@ def cpu_leg(deck, r_vec, steps):
     return t1 - t0
 
 
+def traffic_per_launch():
+    """HBM bytes per step-kernel launch from the committed counter summary
+    (its ``traffic_bytes_per_launch = N`` line), or None."""
+    try:
+        with open(TRAFFIC_SUMMARY) as fh:
+            for line in fh:
+                m = line.strip().strip("`").split("=")
+                if len(m) == 2 and m[0].strip() == "traffic_bytes_per_launch":
+                    return int(m[1].strip().replace(",", ""))
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration():
+    """A plain device-to-device copy of 256 MiB, five times: the kernel whose
+    FETCH_SIZE / WRITE_SIZE readings scale the step kernel's."""
+    import torch
+    a = torch.ones(CAL_BYTES // 8, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": CAL_BYTES, "copies": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,12 +107,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the lattice is not decomposed across GPUs).", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     deck = I.deck(DECK)
     nodes = I.nodes(deck)
@@ -130,7 +168,7 @@ def main():
         "config": {"workload": "D3Q27 BGK sphere 512x256x256 (513x257x257 nodes), radius 20, Re 200, FP64",
                    "nodes": nodes, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch()},
         "cpu_baseline": cpu,
         "kernel": {"name": "k_step<3,double,double,1>", "avg_ms": round(avg_ms, 4),
                    "launches": g["fused_launches"], "algorithmic_bytes_per_launch": bytes_per_launch,

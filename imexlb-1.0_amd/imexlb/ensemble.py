@@ -152,7 +152,8 @@ def main(argv=None):
            "batch_MLUPS": round(nodes * a.steps / t_batch / 1e6, 1),
            "sequential_steps": a.seq_steps, "sequential_s": round(t_seq, 3),
            "sequential_MLUPS": round(nodes * a.seq_steps / t_seq / 1e6, 1),
-           "drag_range": [float(F[:, 0].min()), float(F[:, 0].max())]}
+           "nonfinite_cases": int((~np.isfinite(F).all(axis=1)).sum()),
+           "drag_range": [float(np.nanmin(F[:, 0])), float(np.nanmax(F[:, 0]))]}
     out["speedup"] = round(out["batch_MLUPS"] / out["sequential_MLUPS"], 2)
     print(json.dumps(out), flush=True)
 

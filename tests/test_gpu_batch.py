@@ -7,20 +7,29 @@ import pytest
 import imexlb as I
 from imexlb.ensemble import Batch, cylinder_cases
 
-pytestmark = pytest.mark.gpu
-
-
 def _small_cases():
-    decks = cylinder_cases(5, seed=11)
-    # shrink to test sizes, keeping each case's Re, relaxation and body ratio
+    """Five cylinder cases of uneven size and Reynolds number, small enough to
+    compare with single lattices: Nx = 48..112, Ny = Nx/4, diameter Nx/10,
+    inlet velocity 0.05, Re 20..80 (the relaxation follows from Re)."""
     out = []
-    for k, dk in enumerate(decks):
+    for k in range(5):
         nx = 48 + 16 * k
         ny = nx // 4
-        out.append(I.Deck(2, (nx, ny, 0), dk.Lc, dk.Xpos, dk.Re, dk.ome, 1.0, 1.0, 1.0, 0.01))
+        Lc = 0.1
+        re = 20.0 + 15.0 * k
+        nu = 0.05 * (Lc * nx) / re
+        ome = 1.0 / (3.0 * nu + 0.5)
+        out.append(I.Deck(2, (nx, ny, 0), Lc, 0.25, re, ome, 1.0, 1.0, 1.0, 0.01))
     return out
 
 
+def test_small_cases_are_stable_decks():
+    for dk in _small_cases():
+        d = I.derive(dk)
+        assert abs(d.uu - 0.05) < 1e-12 and 2 * d.radius < dk.n[1] / 2
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("block,cpt,persist", [(64, 1, 0), (256, 2, 1), (512, 4, 0), (128, 4, 1), (256, 1, 0)])
 def test_batch_matches_single_lattices(block, cpt, persist, r_vec):
     decks = _small_cases()
@@ -35,12 +44,14 @@ def test_batch_matches_single_lattices(block, cpt, persist, r_vec):
         lat.advance(37)
         p, u = lat.download_fields()
         pb, ub = b.download_fields(i)
+        assert np.isfinite(p).all() and np.isfinite(u).all(), i
         assert np.array_equal(p, pb) and np.array_equal(u, ub), i
         assert np.array_equal(lat.forces(), Fb[i]), i
         lat.close()
     b.close()
 
 
+@pytest.mark.gpu
 def test_batch_tune_and_continue(r_vec):
     """Tuning picks a valid shape; advancing after it (odd and even step
     counts, through the graph and the single-step path) stays exact."""
@@ -57,11 +68,13 @@ def test_batch_tune_and_continue(r_vec):
     lat.advance(12)
     p, u = lat.download_fields()
     pb, ub = b.download_fields(2)
+    assert np.isfinite(p).all() and np.isfinite(u).all()
     assert np.array_equal(p, pb) and np.array_equal(u, ub)
     lat.close()
     b.close()
 
 
+@pytest.mark.gpu
 def test_batch_fp16_close_to_fp64(r_vec):
     decks = _small_cases()
     b64 = Batch(decks, precision=I.F64)
@@ -72,6 +85,7 @@ def test_batch_fp16_close_to_fp64(r_vec):
     for i in range(len(decks)):
         _, u64 = b64.download_fields(i)
         _, u16 = b16.download_fields(i)
+        assert np.isfinite(u64).all(), i
         err = np.linalg.norm(u16 - u64) / np.linalg.norm(u64)
         assert err < 1e-2, (i, err)
     b64.close()

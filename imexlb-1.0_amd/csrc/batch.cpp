@@ -406,6 +406,13 @@ int imexlb_batch_tune(imexlb_batch *b, int trial_steps, int32_t best[3], double 
     return IMEXLB_OK;
 }
 
+int imexlb_batch_synchronize(imexlb_batch *b)
+{
+    CHECK_BATCH(b);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return IMEXLB_OK;
+}
+
 int imexlb_batch_forces(imexlb_batch *b, double *F)
 {
     CHECK_BATCH(b);

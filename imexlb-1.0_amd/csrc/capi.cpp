@@ -759,15 +759,8 @@ int imexlb_profile_read(imexlb_ctx *c, int64_t launches[3], double ms[3])
 {
     CHECK_DEV(c);
     HIP_TRY(hipStreamSynchronize(c->stream));
-    for (auto &t : c->timed) {
-        float v = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&v, t.a, t.b));
-        c->prof_n[t.cls] += 1;
-        c->prof_ms[t.cls] += double(v);
-        (void)hipEventDestroy(t.a);
-        (void)hipEventDestroy(t.b);
-    }
-    c->timed.clear();
+    int rc = c->harvest(0);
+    if (rc) return rc;
     for (int k = 0; k < 3; ++k) {
         if (launches) launches[k] = c->prof_n[k];
         if (ms) ms[k] = c->prof_ms[k];

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <string>
 #include <vector>
 
@@ -96,6 +97,7 @@ struct imexlb_ctx {
     // profiling
     bool prof = false;
     std::vector<TimedLaunch> timed;
+    static constexpr size_t kMaxTimed = 512;
     int64_t prof_n[3] = {0, 0, 0};
     double prof_ms[3] = {0, 0, 0};
 
@@ -128,6 +130,28 @@ struct imexlb_ctx {
         HIP_TRY(hipEventCreate(&e));
         HIP_TRY(hipEventRecord(e, stream));
         timed.push_back(TimedLaunch{cls, b, e});
+        // bound the live event pairs of a long timed run: fold the oldest
+        // half into the totals (waiting for them) once kMaxTimed are held
+        if (timed.size() >= kMaxTimed) return harvest(kMaxTimed / 2);
+        return IMEXLB_OK;
+    }
+
+    // add the elapsed time of all but the newest `keep` event pairs to
+    // prof_n / prof_ms and destroy them (waits for those launches)
+    int harvest(size_t keep)
+    {
+        const size_t n = timed.size() > keep ? timed.size() - keep : 0;
+        for (size_t k = 0; k < n; ++k) {
+            TimedLaunch &t = timed[k];
+            HIP_TRY(hipEventSynchronize(t.b));
+            float v = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&v, t.a, t.b));
+            prof_n[t.cls] += 1;
+            prof_ms[t.cls] += double(v);
+            (void)hipEventDestroy(t.a);
+            (void)hipEventDestroy(t.b);
+        }
+        timed.erase(timed.begin(), timed.begin() + std::ptrdiff_t(n));
         return IMEXLB_OK;
     }
 

@@ -1,20 +1,24 @@
 // lbm_kernels.hip -- gfx950 kernels of the lattice Boltzmann time step.
 //
-// Device layout: populations are structure-of-arrays, one plane of `plane`
-// values per direction q, over a padded grid with ONE ghost layer:
-//     slot(q, x, y, z) = q*plane + o0 + x + Px*y + PxPy*z,  x in [-1, nx+1] ...
+// Device layout: populations live on a padded grid with ONE ghost layer,
+// padded cell c = o0 + x + Px*y + PxPy*z (x in [-1, nx+1] ...), stored in
+// tiles of IMEXLB_TILE consecutive cells with the Q directions of a tile one
+// after another: slot(q, c) = ((c/T)*Q + q)*T + c%T (T = 0 selects plain
+// planes, slot = q*plane + c).
 // Interior fluid nodes hold post-collision values; interior solid nodes
 // hold 0 (the reference never writes them, lbm.f90:57); the ghost layer and
 // the body cells the reference rewrites (its b_user list) are filled once
 // per step from a resolved table (k_fill), which reproduces PassF plus the
 // five copy phases of BoundaryCondition (lbm.f90:96-155) slot for slot.
 //
-// One thread per interior node, x fastest.  In FP64 form every kernel keeps
+// One thread per padded cell (threads on ghost cells leave), x fastest, so a
+// wave covers one tile.  In FP64 form every kernel keeps
 // the reference's operation order (lbm.f90:56-74, 163-199) and the library
 // is compiled without floating-point contraction, so results are the
 // reference's IEEE-754 results.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <type_traits>
 
@@ -36,7 +40,7 @@
 #endif
 // thread i handles padded cell i (1) or interior node i (0)
 #ifndef IMEXLB_CELLMAP
-#define IMEXLB_CELLMAP 0
+#define IMEXLB_CELLMAP 1
 #endif
 
 namespace imexlb {
@@ -507,18 +511,36 @@ __global__ __launch_bounds__(256) void k_fill(S *__restrict__ buf, const S *__re
 
 // Monitor (lbm.f90:212-217): max of ux^2+uy^2 over fluid nodes except the
 // first fluid node in scan order; sqrt is taken on the host (monotone, so
-// the maximum of the roots equals the root of the maximum).
+// the maximum of the roots equals the root of the maximum).  A bounded grid
+// strides over the nodes with MON_UNROLL independent loads in flight per
+// thread; one 64-bit atomicMax per block (non-negative doubles order like
+// their bit patterns).
+constexpr unsigned MON_BLOCKS = 1024;
+constexpr int MON_UNROLL = 4;
+
 template <int D>
 __global__ __launch_bounds__(256) void k_monitor(const double *__restrict__ ux, const double *__restrict__ uy,
                                                  const uint8_t *__restrict__ geo, uint32_t skip, KParams kp,
                                                  unsigned long long *__restrict__ out)
 {
     __shared__ double red[4];
-    int x, y, z;
-    uint32_t c;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     double v = 0.0;
-    if (decode<D>(kp, blockIdx.x * blockDim.x + threadIdx.x, x, y, z, c) && geo[c] && c != skip)
-        v = ux[c] * ux[c] + uy[c] * uy[c];
+    for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; base < kp.N;
+         base += MON_UNROLL * stride) {
+        double a[MON_UNROLL];
+#pragma unroll
+        for (int k = 0; k < MON_UNROLL; ++k) {
+            const uint64_t i = base + uint64_t(k) * stride;
+            int x, y, z;
+            uint32_t c;
+            a[k] = 0.0;
+            if (i < kp.N && decode<D>(kp, uint32_t(i), x, y, z, c) && geo[c] && c != skip)
+                a[k] = ux[c] * ux[c] + uy[c] * uy[c];
+        }
+#pragma unroll
+        for (int k = 0; k < MON_UNROLL; ++k) v = fmax(v, a[k]);
+    }
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) red[wave] = v;
@@ -698,12 +720,11 @@ hipError_t launch_forces(int dim, int prec, const void *buf, const int64_t *out_
 hipError_t launch_monitor(int dim, const StepArgs &a, uint32_t skip, unsigned long long *out, hipStream_t s)
 {
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, out);
+    const unsigned nb = std::min(nblocks(a.kp.N), MON_BLOCKS);
     if (dim == 2)
-        hipLaunchKernelGGL((k_monitor<2>), dim3(nblocks(a.kp.N)), dim3(256), 0, s, a.ux, a.uy, a.geo, skip, a.kp,
-                           out);
+        hipLaunchKernelGGL((k_monitor<2>), dim3(nb), dim3(256), 0, s, a.ux, a.uy, a.geo, skip, a.kp, out);
     else
-        hipLaunchKernelGGL((k_monitor<3>), dim3(nblocks(a.kp.N)), dim3(256), 0, s, a.ux, a.uy, a.geo, skip, a.kp,
-                           out);
+        hipLaunchKernelGGL((k_monitor<3>), dim3(nb), dim3(256), 0, s, a.ux, a.uy, a.geo, skip, a.kp, out);
     return hipGetLastError();
 }
 

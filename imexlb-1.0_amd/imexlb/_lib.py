@@ -59,6 +59,7 @@ SIGNATURES = {
     "imexlb_batch_advance": (C.c_int, [_P, C.c_int64]),
     "imexlb_batch_set_config": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
     "imexlb_batch_tune": (C.c_int, [_P, C.c_int, _I32P, _DP]),
+    "imexlb_batch_synchronize": (C.c_int, [_P]),
     "imexlb_batch_forces": (C.c_int, [_P, _DP]),
     "imexlb_batch_download_fields": (C.c_int, [_P, C.c_int, _DP, _DP]),
     "imexlb_batch_info": (C.c_int, [_P, _I64P]),

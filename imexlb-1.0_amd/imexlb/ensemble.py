@@ -86,7 +86,7 @@ class Batch:
         return p, u
 
     def synchronize(self):
-        self.forces()  # synchronises the batch stream
+        check(lib().imexlb_batch_synchronize(self._h), "imexlb_batch_synchronize")
 
 
 def cylinder_cases(ncase, seed=7, uu=0.05, xpos=0.2):

@@ -197,6 +197,8 @@ int imexlb_batch_set_config(imexlb_batch *b, int block, int cells_per_thread, in
 /* Times every shape over trial_steps steps (advancing the ensemble) and
  * keeps the fastest; best = (block, cells_per_thread, persistent). */
 int imexlb_batch_tune(imexlb_batch *b, int trial_steps, int32_t best[3], double *best_ms_per_step);
+/* Waits for every step queued on the batch's stream. */
+int imexlb_batch_synchronize(imexlb_batch *b);
 /* Momentum-exchange force of every case, F[3*i..3*i+2]. */
 int imexlb_batch_forces(imexlb_batch *b, double *F);
 /* p and u of case i, interior layout (u[d][y][x]). */

@@ -58,6 +58,7 @@ def test_batch_tune_and_continue(r_vec):
     decks = _small_cases()
     b = Batch(decks, precision=I.F64)
     b.init(r_vec)
+    b.synchronize()  # valid before any step
     best = b.tune(4)
     assert best["block"] in (64, 128, 256, 512) and best["cells_per_thread"] in (1, 2, 4)
     assert best["ms_per_step"] > 0

@@ -27,7 +27,6 @@
 
 namespace {
 
-
 #define CHECK_CTX(c)                                                       \
     do {                                                                   \
         if (!(c)) return fail(IMEXLB_E_ARG, "null context");              \
@@ -210,7 +209,9 @@ int imexlb_create(imexlb_ctx **out, int dim, int nx, int ny, int nz, int device,
     c->Pz = dim == 3 ? int64_t(nz) + 3 : 1;
     c->PxPy = c->Px * c->Py;
     c->o0 = 1 + c->Px + (dim == 3 ? c->PxPy : 0);
-    c->plane = ((c->Px * c->Py * c->Pz + 127) / 128) * 128;  // whole tiles in the tiled layout
+    // whole tiles in the tiled layout (and 128-value alignment of planes)
+    constexpr int64_t kRound = IMEXLB_TILE > 128 ? IMEXLB_TILE : 128;
+    c->plane = ((c->Px * c->Py * c->Pz + kRound - 1) / kRound) * kRound;
     const int64_t nodes = int64_t(nx + 1) * (ny + 1) * (dim == 3 ? nz + 1 : 1);
     if (c->plane >= (int64_t(1) << 31) || nodes >= (int64_t(1) << 31))
         return fail(IMEXLB_E_ARG, "grid too large for 32-bit node indexing");
@@ -285,6 +286,7 @@ int imexlb_set_params(imexlb_ctx *c, double ome, double rho0, double uu)
         return fail(IMEXLB_E_ARG, "non-finite parameter or rho0 == 0");
     c->ome = c->kp.ome = ome;
     c->rho0 = c->kp.rho0 = rho0;
+    c->kp.inv_rho0 = 1.0 / rho0;
     c->uu = c->kp.uu = uu;
     if (!c->host_only) {
         HIP_TRY(hipSetDevice(c->device));

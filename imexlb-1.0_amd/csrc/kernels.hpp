@@ -33,6 +33,7 @@ struct FastDiv {
 #ifndef IMEXLB_TILE
 #define IMEXLB_TILE 64
 #endif
+static_assert(IMEXLB_TILE >= 0 && (IMEXLB_TILE & (IMEXLB_TILE - 1)) == 0, "tile size: 0 or a power of two");
 
 // Geometry and scalars every kernel needs (passed by value).
 struct KParams {
@@ -49,9 +50,10 @@ struct KParams {
     int64_t Py;          // padded rows per z plane
     FastDiv div_px;      // / Px
     FastDiv div_py;      // / Py
+    double inv_rho0;     // 1/rho0 (FP32-compute forms multiply by it)
 };
 
-// slot of direction q at padded cell c, host side (kernels use pslot)
+// slot of direction q at padded cell c, host side (kernels use cslot)
 inline int64_t pop_slot(int64_t q, int64_t c, int64_t plane, int Q)
 {
 #if IMEXLB_TILE

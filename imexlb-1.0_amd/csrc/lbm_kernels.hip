@@ -38,6 +38,11 @@
 #ifndef IMEXLB_XCD_REMAP
 #define IMEXLB_XCD_REMAP 1
 #endif
+// minimum waves per SIMD the step kernel is compiled for (0: compiler's
+// choice); it caps the registers per thread at 512/waves
+#ifndef IMEXLB_STEP_WAVES
+#define IMEXLB_STEP_WAVES 0
+#endif
 // thread i handles padded cell i (1) or interior node i (0)
 #ifndef IMEXLB_CELLMAP
 #define IMEXLB_CELLMAP 1
@@ -201,14 +206,109 @@ __device__ __forceinline__ int64_t shift(const KParams &kp, int q)
 
 // Population slot of direction q at padded cell c: planes (one per q) or
 // tiles of IMEXLB_TILE cells holding all directions of the tile together.
+// Unsigned cell index: no sign fix-ups in the tile division, and the slot
+// comes out of one 64-bit multiply-add.
 template <int Q>
-__device__ __forceinline__ int64_t pslot(const KParams &kp, int q, int64_t c)
+__device__ __forceinline__ uint64_t cslot(const KParams &kp, uint32_t q, uint32_t cell)
 {
 #if IMEXLB_TILE
-    return ((c / IMEXLB_TILE) * Q + q) * IMEXLB_TILE + (c % IMEXLB_TILE);
+    constexpr uint32_t T = IMEXLB_TILE;
+    return uint64_t(cell / T) * (uint64_t(Q) * T) + ((cell % T) | (q * T));
 #else
-    return int64_t(q) * kp.plane + c;
+    return uint64_t(q) * uint64_t(kp.plane) + cell;
 #endif
+}
+
+// padded-cell offset of the neighbour a population of direction q comes
+// from, e_q in cells (|offset| < 2^31)
+template <int D>
+__device__ __forceinline__ uint32_t shift32(const KParams &kp, int q)
+{
+    using L = Lat<D>;
+    return uint32_t(L::ex(q) + int32_t(kp.Px) * L::ey(q) + int32_t(kp.PxPy) * L::ez(q));
+}
+
+// ---------------------------------------------------------------------------
+// FP32-compute forms (FP16/BF16/FP32 storage): the same quantities with
+// fewer instructions -- no bit-for-bit constraint applies, so sums start at
+// their first term, divisions become multiplies by reciprocals and products
+// fuse into FMAs.
+
+// sum of f[q] over the directions where pick(q) holds (0 if none)
+template <int Q, class Pick>
+__device__ __forceinline__ float sum_where(const float (&f)[Q], Pick pick)
+{
+    float acc = 0.0f;
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        if (pick(q)) {
+            acc = any ? acc + f[q] : f[q];
+            any = true;
+        }
+    return acc;
+}
+
+template <int D>
+__device__ __forceinline__ void moments_f32(const float (&f)[Lat<D>::Q], const KParams &kp, int x, int y, int z,
+                                            float &p, float &ux, float &uy, float &uz)
+{
+    using L = Lat<D>;
+    const float ps = sum_where<L::Q>(f, [](int) { return true; });
+    const float sx = sum_where<L::Q>(f, [](int q) { return L::ex(q) > 0; }) -
+                     sum_where<L::Q>(f, [](int q) { return L::ex(q) < 0; });
+    const float sy = sum_where<L::Q>(f, [](int q) { return L::ey(q) > 0; }) -
+                     sum_where<L::Q>(f, [](int q) { return L::ey(q) < 0; });
+    const float ir = float(kp.inv_rho0);
+    p = ps * (1.0f / 3.0f);
+    ux = sx * ir;
+    uy = sy * ir;
+    uz = 0.0f;
+    if (D == 3)
+        uz = (sum_where<L::Q>(f, [](int q) { return L::ez(q) > 0; }) -
+              sum_where<L::Q>(f, [](int q) { return L::ez(q) < 0; })) *
+             ir;
+    if (x == 0) {
+        const int h = D == 2 ? y : z;
+        const int nh = D == 2 ? kp.ny : kp.nz;
+        ux = float(kp.uu) * float(kp.rho0) * 4.0f * float(h) * float(nh - h) / float(nh * nh);
+        uy = 0.0f;
+    }
+    if (x == kp.nx) p = 0.0f;
+}
+
+// feq_q = w_q (3p + rho0 (3 e.u + 4.5 (e.u)^2 - 1.5 u.u)) and
+// f' = (1-ome) f + ome feq as (1-ome) f + (ome w_q rho0) t + ome w_q base with
+// t = e.u (3 + 4.5 e.u) and base = 3p - 1.5 rho0 u.u; the per-weight
+// products are shared between directions of equal weight.
+template <int D, class Emit>
+__device__ __forceinline__ void collide_f32(const float (&f)[Lat<D>::Q], float p, float ux, float uy, float uz,
+                                            const KParams &kp, Emit emit)
+{
+    using L = Lat<D>;
+    const float ome = float(kp.ome), rho0 = float(kp.rho0);
+    const float omc = 1.0f - ome;
+    float udu = ux * ux;
+    udu = __builtin_fmaf(uy, uy, udu);
+    if (D == 3) udu = __builtin_fmaf(uz, uz, udu);
+    const float base = __builtin_fmaf(-1.5f * rho0, udu, 3.0f * p);
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+        const float ow = ome * float(L::w(q));
+        float edu = 0.0f;
+        bool first = true;
+        auto term = [&](int e, float v) {
+            if (e == 0) return;
+            edu = first ? (e > 0 ? v : -v) : (e > 0 ? edu + v : edu - v);
+            first = false;
+        };
+        term(L::ex(q), ux);
+        term(L::ey(q), uy);
+        if (D == 3) term(L::ez(q), uz);
+        float eq = ow * base;
+        if (!first) eq = __builtin_fmaf(ow * rho0, edu * __builtin_fmaf(4.5f, edu, 3.0f), eq);
+        emit(q, __builtin_fmaf(omc, f[q], eq));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -220,6 +320,10 @@ __device__ __forceinline__ void moments(const C (&f)[Lat<D>::Q], const KParams &
                                         C &ux, C &uy, C &uz)
 {
     using L = Lat<D>;
+    if constexpr (std::is_same<C, float>::value) {
+        moments_f32<D>(f, kp, x, y, z, p, ux, uy, uz);
+        return;
+    }
     C ps = C(0), sx = C(0), sy = C(0), sz = C(0);
 #pragma unroll
     for (int q = 0; q < L::Q; ++q) {
@@ -253,6 +357,10 @@ __device__ __forceinline__ void collide(const C (&f)[Lat<D>::Q], C p, C ux, C uy
                                         Emit emit)
 {
     using L = Lat<D>;
+    if constexpr (std::is_same<C, float>::value) {
+        collide_f32<D>(f, p, ux, uy, uz, kp, emit);
+        return;
+    }
     const C ome = C(kp.ome), rho0 = C(kp.rho0);
     const C omc = C(1) - ome;
     C udu = ux * ux + uy * uy;
@@ -281,6 +389,20 @@ __device__ __forceinline__ void collide(const C (&f)[Lat<D>::Q], C p, C ux, C uy
 }
 
 // ---------------------------------------------------------------------------
+// Order in which the gather issues its loads: q order (0), or grouped by the
+// source row (y, z shift), x shift fastest (1), so consecutive loads of a
+// wave fall in the same stretch of memory.  Arithmetic keeps q order.
+#ifndef IMEXLB_LOAD_ORDER
+#define IMEXLB_LOAD_ORDER 0
+#endif
+template <int D>
+__device__ __forceinline__ constexpr int load_order(int j)
+{
+    if (!IMEXLB_LOAD_ORDER) return j;
+    return D == 2 ? (j % 3) * 3 + j / 3 : (j / 9) * 9 + (j % 3) * 3 + (j % 9) / 3;
+}
+
+// ---------------------------------------------------------------------------
 // The step kernel.  MODE bits:
 //   K_COLLIDE  relax and write dst (the next step's post-collision state)
 //   K_WRITE_F  write the gathered populations (array f)
@@ -303,8 +425,9 @@ __device__ __forceinline__ void step_node(const S *__restrict__ src, S *__restri
     const uint8_t fluid = geo[c];
     C f[L::Q];
 #pragma unroll
-    for (int q = 0; q < L::Q; ++q) {
-        const S *a = src + pslot<L::Q>(kp, q, int64_t(c) - ((MODE & K_NOSHIFT) ? 0 : shift<D>(kp, q)));
+    for (int j = 0; j < L::Q; ++j) {
+        const int q = load_order<D>(j);
+        const S *a = src + cslot<L::Q>(kp, q, (MODE & K_NOSHIFT) ? c : c - shift32<D>(kp, q));
         f[q] = Store<S>::ld(IMEXLB_STEP_NTLOAD ? nt_load(a) : *a);
     }
     if (!fluid) {
@@ -321,7 +444,7 @@ __device__ __forceinline__ void step_node(const S *__restrict__ src, S *__restri
     }
     if (MODE & K_WRITE_F) {
 #pragma unroll
-        for (int q = 0; q < L::Q; ++q) fout[pslot<L::Q>(kp, q, c)] = Store<S>::template st<C>(f[q]);
+        for (int q = 0; q < L::Q; ++q) fout[cslot<L::Q>(kp, q, c)] = Store<S>::template st<C>(f[q]);
     }
     if (!(MODE & (K_WRITE_PU | K_COLLIDE))) return;
     C p, ux, uy, uz;
@@ -334,7 +457,7 @@ __device__ __forceinline__ void step_node(const S *__restrict__ src, S *__restri
     }
     if (MODE & K_COLLIDE) {
         collide<D, C>(f, p, ux, uy, uz, kp, [&](int q, C v) {
-            S *o = dst + pslot<L::Q>(kp, q, c);
+            S *o = dst + cslot<L::Q>(kp, q, c);
             if (IMEXLB_STEP_NT)
                 nt_store(o, Store<S>::template st<C>(v));
             else
@@ -343,8 +466,14 @@ __device__ __forceinline__ void step_node(const S *__restrict__ src, S *__restri
     }
 }
 
+#if IMEXLB_STEP_WAVES
+#define IMEXLB_STEP_BOUNDS __launch_bounds__(IMEXLB_STEP_BLOCK, IMEXLB_STEP_WAVES)
+#else
+#define IMEXLB_STEP_BOUNDS __launch_bounds__(IMEXLB_STEP_BLOCK)
+#endif
+
 template <int D, class S, class C, int MODE>
-__global__ __launch_bounds__(IMEXLB_STEP_BLOCK) void k_step(const S *__restrict__ src, S *__restrict__ dst,
+__global__ IMEXLB_STEP_BOUNDS void k_step(const S *__restrict__ src, S *__restrict__ dst,
                                               S *__restrict__ fout, double *__restrict__ pout,
                                               double *__restrict__ uxo, double *__restrict__ uyo,
                                               double *__restrict__ uzo, const uint8_t *__restrict__ geo,
@@ -425,9 +554,9 @@ __global__ __launch_bounds__(256) void k_collide(const S *__restrict__ fin, S *_
     if (!geo[c]) return;
     C f[L::Q];
 #pragma unroll
-    for (int q = 0; q < L::Q; ++q) f[q] = Store<S>::ld(fin[pslot<L::Q>(kp, q, c)]);
+    for (int q = 0; q < L::Q; ++q) f[q] = Store<S>::ld(fin[cslot<L::Q>(kp, q, c)]);
     collide<D, C>(f, C(p[c]), C(ux[c]), C(uy[c]), D == 3 ? C(uz[c]) : C(0), kp,
-                  [&](int q, C v) { dst[pslot<L::Q>(kp, q, c)] = Store<S>::template st<C>(v); });
+                  [&](int q, C v) { dst[cslot<L::Q>(kp, q, c)] = Store<S>::template st<C>(v); });
 }
 
 // InitPDF (lbm.f90:29-47): f = feq(p, u) on fluid nodes.
@@ -463,7 +592,7 @@ __global__ __launch_bounds__(256) void k_init_pdf(S *__restrict__ fout, const do
         term(L::ey(q), u1);
         if (D == 3) term(L::ez(q), u2);
         const C feq = C(L::w(q)) * (pp * C(3) + rho0 * ((C(3) * edu + C(4.5) * (edu * edu)) - C(1.5) * udu));
-        fout[pslot<L::Q>(kp, q, c)] = Store<S>::template st<C>(feq);
+        fout[cslot<L::Q>(kp, q, c)] = Store<S>::template st<C>(feq);
     }
 }
 

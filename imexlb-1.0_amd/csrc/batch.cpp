@@ -30,7 +30,8 @@ struct imexlb_batch {
     void *f = nullptr;
     double *p = nullptr, *ux = nullptr, *uy = nullptr, *rv = nullptr;
     uint8_t *geo = nullptr;
-    int64_t *fill_dst = nullptr, *fill_src = nullptr;
+    void *fill_dst = nullptr, *fill_src = nullptr;  // int32 (fill32) or int64 slots
+    bool fill32 = false;
     int64_t nfill = 0;
     int64_t *link_out = nullptr, *link_back = nullptr;
     int8_t *link_q = nullptr;
@@ -114,7 +115,7 @@ struct imexlb_batch {
 
     hipError_t enqueue_step(int b)
     {
-        hipError_t e = launch_batch_fill(prec, fb[b], fb[1 - b], fill_dst, fill_src, nfill, queue, stream);
+        hipError_t e = launch_batch_fill(prec, fb[b], fb[1 - b], fill_dst, fill_src, fill32, nfill, queue, stream);
         if (e != hipSuccess) return e;
         return launch_batch_step(prec, block, cpt, persist, persist_grid, fb[b], fb[1 - b], geo, d_cases, tile_case,
                                  tile_first, ntiles, queue, stream);
@@ -264,11 +265,13 @@ int imexlb_batch_create(imexlb_batch **out, int ncase, const int32_t *nxny, cons
     const size_t nblk = (size_t(std::max<int64_t>(b->max_links, 1)) + 255) / 256;
     imexlb_batch *B = b.get();
     int rc;
+    B->fill32 = fill_fits_32(B->pop_total);
+    const size_t isz = B->fill32 ? 4 : 8;
     if ((rc = balloc(B, &B->fb[0], pop)) || (rc = balloc(B, &B->fb[1], pop)) || (rc = balloc(B, &B->f, pop)) ||
         (rc = balloc(B, (void **)&B->p, fld)) || (rc = balloc(B, (void **)&B->ux, fld)) ||
         (rc = balloc(B, (void **)&B->uy, fld)) || (rc = balloc(B, (void **)&B->rv, 100 * sizeof(double))) ||
         (rc = balloc(B, (void **)&B->geo, size_t(B->cell_total))) ||
-        (rc = balloc(B, (void **)&B->fill_dst, nf * 8)) || (rc = balloc(B, (void **)&B->fill_src, nf * 8)) ||
+        (rc = balloc(B, &B->fill_dst, nf * isz)) || (rc = balloc(B, &B->fill_src, nf * isz)) ||
         (rc = balloc(B, (void **)&B->link_out, nl * 8)) || (rc = balloc(B, (void **)&B->link_back, nl * 8)) ||
         (rc = balloc(B, (void **)&B->link_q, nl)) ||
         (rc = balloc(B, (void **)&B->force_partial, nblk * 3 * sizeof(double))) ||
@@ -277,10 +280,8 @@ int imexlb_batch_create(imexlb_batch **out, int ncase, const int32_t *nxny, cons
         (rc = balloc(B, (void **)&B->queue, 64)))
         return rc;
     HIP_TRY(hipMemcpy(B->geo, geo.data(), geo.size(), hipMemcpyHostToDevice));
-    if (!fd.empty()) {
-        HIP_TRY(hipMemcpy(B->fill_dst, fd.data(), fd.size() * 8, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(B->fill_src, fs.data(), fs.size() * 8, hipMemcpyHostToDevice));
-    }
+    HIP_TRY(upload_slots(B->fill_dst, fd, B->fill32));
+    HIP_TRY(upload_slots(B->fill_src, fs, B->fill32));
     if (!lo.empty()) {
         HIP_TRY(hipMemcpy(B->link_out, lo.data(), lo.size() * 8, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(B->link_back, lbk.data(), lbk.size() * 8, hipMemcpyHostToDevice));

@@ -137,6 +137,19 @@ int build_device_tables(const imexlb_ctx *c, DeviceTables &t)
         t.fill_dst.push_back(d);
         t.fill_src.push_back(s);
     }
+    // destination order: consecutive fill threads write consecutive slots
+    {
+        std::vector<size_t> order(t.fill_dst.size());
+        for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+        std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return t.fill_dst[a] < t.fill_dst[b]; });
+        std::vector<int64_t> d(order.size()), s(order.size());
+        for (size_t k = 0; k < order.size(); ++k) {
+            d[k] = t.fill_dst[order[k]];
+            s[k] = t.fill_src[order[k]];
+        }
+        t.fill_dst.swap(d);
+        t.fill_src.swap(s);
+    }
     // body links: fluid node x, direction q, interior solid neighbour x+e_q
     t.link_out.clear();
     t.link_back.clear();
@@ -324,16 +337,16 @@ int imexlb_set_geometry(imexlb_ctx *c, double pos, double radius)
     c->nlink = int64_t(t.link_out.size());
     const size_t nf = std::max<size_t>(t.fill_dst.size(), 1), nl = std::max<size_t>(t.link_out.size(), 1);
     const size_t nblk = (nl + 255) / 256;
-    if ((rc = dev_alloc(c, (void **)&c->fill_dst, nf * 8)) || (rc = dev_alloc(c, (void **)&c->fill_src, nf * 8)) ||
+    c->fill32 = fill_fits_32(int64_t(c->Q) * c->plane);
+    const size_t isz = c->fill32 ? 4 : 8;
+    if ((rc = dev_alloc(c, &c->fill_dst, nf * isz)) || (rc = dev_alloc(c, &c->fill_src, nf * isz)) ||
         (rc = dev_alloc(c, (void **)&c->link_out, nl * 8)) || (rc = dev_alloc(c, (void **)&c->link_back, nl * 8)) ||
         (rc = dev_alloc(c, (void **)&c->link_q, nl)) ||
         (rc = dev_alloc(c, (void **)&c->force_partial, nblk * 3 * sizeof(double))) ||
         (rc = dev_alloc(c, (void **)&c->force_out, 4 * sizeof(double))))
         return rc;
-    if (c->nfill) {
-        HIP_TRY(hipMemcpy(c->fill_dst, t.fill_dst.data(), t.fill_dst.size() * 8, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(c->fill_src, t.fill_src.data(), t.fill_src.size() * 8, hipMemcpyHostToDevice));
-    }
+    HIP_TRY(upload_slots(c->fill_dst, t.fill_dst, c->fill32));
+    HIP_TRY(upload_slots(c->fill_src, t.fill_src, c->fill32));
     if (c->nlink) {
         HIP_TRY(hipMemcpy(c->link_out, t.link_out.data(), t.link_out.size() * 8, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(c->link_back, t.link_back.data(), t.link_back.size() * 8, hipMemcpyHostToDevice));
@@ -568,7 +581,7 @@ static int pair_graph(imexlb_ctx *c, hipGraphExec_t *out)
     hipError_t e = hipSuccess;
     for (int k = 0; k < 2 && e == hipSuccess; ++k) {
         const int b = k == 0 ? 1 - X : X;
-        e = launch_fill(c->prec, c->fb[b], c->fb[1 - b], c->fill_dst, c->fill_src, c->nfill, c->stream);
+        e = launch_fill(c->prec, c->fb[b], c->fb[1 - b], c->fill_dst, c->fill_src, c->fill32, c->nfill, c->stream);
         if (e != hipSuccess) break;
         a.src = c->fb[b];
         a.dst = c->fb[1 - b];

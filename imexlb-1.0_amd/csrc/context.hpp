@@ -65,7 +65,8 @@ struct imexlb_ctx {
     void *f = nullptr;
     double *p = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr, *rv = nullptr;
     uint8_t *geo = nullptr;
-    int64_t *fill_dst = nullptr, *fill_src = nullptr;
+    void *fill_dst = nullptr, *fill_src = nullptr;  // int32 (fill32) or int64 slots
+    bool fill32 = false;
     int64_t nfill = 0;
     unsigned long long *red = nullptr;
     // body links for the momentum-exchange force
@@ -171,7 +172,9 @@ struct imexlb_ctx {
     // not use)
     int run_fill(int buf)
     {
-        return launch(1, [&] { return launch_fill(prec, fb[buf], fb[1 - buf], fill_dst, fill_src, nfill, stream); });
+        return launch(1, [&] {
+            return launch_fill(prec, fb[buf], fb[1 - buf], fill_dst, fill_src, fill32, nfill, stream);
+        });
     }
 
     int run_step(int mode, const void *src, void *dst, void *fout)
@@ -253,5 +256,18 @@ struct DeviceTables {
     std::vector<int8_t> link_q;
 };
 int build_device_tables(const imexlb_ctx *c, DeviceTables &t);
+
+// The boundary fill's slot tables fit int32 entries when every slot and its
+// previous-step encoding (-slot-2) do.
+inline bool fill_fits_32(int64_t nslots) { return nslots < (int64_t(1) << 31) - 2; }
+
+// copy a slot table to the device as int32 or int64 entries
+inline hipError_t upload_slots(void *dev, const std::vector<int64_t> &v, bool idx32)
+{
+    if (v.empty()) return hipSuccess;
+    if (!idx32) return hipMemcpy(dev, v.data(), v.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+    std::vector<int32_t> n(v.begin(), v.end());
+    return hipMemcpy(dev, n.data(), n.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+}
 
 }  // namespace imexlb

@@ -89,8 +89,9 @@ hipError_t launch_step(int dim, int prec, int mode, const StepArgs &a, hipStream
 hipError_t launch_collide(int dim, int prec, const StepArgs &a, hipStream_t s);
 hipError_t launch_init_pdf(int dim, int prec, const StepArgs &a, hipStream_t s);
 hipError_t launch_init_up(int dim, const StepArgs &a, const double *rv, hipStream_t s);
-hipError_t launch_fill(int prec, void *buf, const void *prev, const int64_t *dslot, const int64_t *sslot, int64_t n,
-                       hipStream_t s);
+// Boundary fill; the slot tables hold int32 (idx32) or int64 entries.
+hipError_t launch_fill(int prec, void *buf, const void *prev, const void *dslot, const void *sslot, bool idx32,
+                       int64_t n, hipStream_t s);
 hipError_t launch_monitor(int dim, const StepArgs &a, uint32_t skip, unsigned long long *out, hipStream_t s);
 // Batched ensemble (D2Q9; prec 0, 1 or 2).  block in {64,128,256,512},
 // cpt in {1,2,4}; persist launches `grid` blocks that take tiles from
@@ -98,8 +99,8 @@ hipError_t launch_monitor(int dim, const StepArgs &a, uint32_t skip, unsigned lo
 hipError_t launch_batch_step(int prec, int block, int cpt, int persist, unsigned grid, const void *src, void *dst,
                              const uint8_t *geo, const BatchCase *cases, const uint32_t *tile_case,
                              const uint32_t *tile_first, uint32_t ntiles, uint32_t *queue, hipStream_t s);
-hipError_t launch_batch_fill(int prec, void *buf, const void *prev, const int64_t *dslot, const int64_t *sslot,
-                             int64_t n, uint32_t *queue, hipStream_t s);
+hipError_t launch_batch_fill(int prec, void *buf, const void *prev, const void *dslot, const void *sslot,
+                             bool idx32, int64_t n, uint32_t *queue, hipStream_t s);
 // body-link momentum exchange; partial holds 3 doubles per 256 links (at least 3)
 hipError_t launch_forces(int dim, int prec, const void *buf, const int64_t *out_slot, const int64_t *back_slot,
                          const int8_t *lq, int64_t n, double *partial, double *out, hipStream_t s);

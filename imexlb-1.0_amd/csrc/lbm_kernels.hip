@@ -526,11 +526,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch_step(const S *__restrict__ src,
 
 // Boundary fill of all cases (slots already offset per case); thread 0 also
 // rewinds the tile queue of the step kernel that follows on the stream.
-template <class S>
+template <class S, class I>
 __global__ __launch_bounds__(256) void k_batch_fill(S *__restrict__ buf, const S *__restrict__ prev,
-                                                    const int64_t *__restrict__ dslot,
-                                                    const int64_t *__restrict__ sslot, int64_t n,
-                                                    uint32_t *__restrict__ queue)
+                                                    const I *__restrict__ dslot, const I *__restrict__ sslot,
+                                                    int64_t n, uint32_t *__restrict__ queue)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i == 0) *queue = 0u;
@@ -626,11 +625,11 @@ __global__ __launch_bounds__(256) void k_init_up(double *__restrict__ p, double 
 // Boundary fill of this step's buffer: dst slot <- this step's fluid slot
 // (s >= 0), 0 (s == -1), or a body-cell slot of the previous step's buffer
 // (s <= -2, slot -s-2).  Sources are never destinations, so the order of
-// the entries does not matter.
-template <class S>
+// the entries does not matter; they are sorted by destination, so a wave's
+// writes land on consecutive slots.  I: int32_t or int64_t slot tables.
+template <class S, class I>
 __global__ __launch_bounds__(256) void k_fill(S *__restrict__ buf, const S *__restrict__ prev,
-                                              const int64_t *__restrict__ dslot, const int64_t *__restrict__ sslot,
-                                              int64_t n)
+                                              const I *__restrict__ dslot, const I *__restrict__ sslot, int64_t n)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -815,14 +814,18 @@ hipError_t launch_init_up(int dim, const StepArgs &a, const double *rv, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_fill(int prec, void *buf, const void *prev, const int64_t *dslot, const int64_t *sslot, int64_t n,
-                       hipStream_t s)
+hipError_t launch_fill(int prec, void *buf, const void *prev, const void *dslot, const void *sslot, bool idx32,
+                       int64_t n, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     return dispatch_prec(prec, [&](auto *sp, auto *) {
         using S = std::remove_pointer_t<decltype(sp)>;
-        hipLaunchKernelGGL((k_fill<S>), dim3(nblocks(uint64_t(n))), dim3(256), 0, s, (S *)buf, (const S *)prev, dslot,
-                           sslot, n);
+        if (idx32)
+            hipLaunchKernelGGL((k_fill<S, int32_t>), dim3(nblocks(uint64_t(n))), dim3(256), 0, s, (S *)buf,
+                               (const S *)prev, (const int32_t *)dslot, (const int32_t *)sslot, n);
+        else
+            hipLaunchKernelGGL((k_fill<S, int64_t>), dim3(nblocks(uint64_t(n))), dim3(256), 0, s, (S *)buf,
+                               (const S *)prev, (const int64_t *)dslot, (const int64_t *)sslot, n);
         return hipGetLastError();
     });
 }
@@ -927,14 +930,18 @@ hipError_t launch_batch_step(int prec, int block, int cpt, int persist, unsigned
     }
 }
 
-hipError_t launch_batch_fill(int prec, void *buf, const void *prev, const int64_t *dslot, const int64_t *sslot,
-                             int64_t n, uint32_t *queue, hipStream_t s)
+hipError_t launch_batch_fill(int prec, void *buf, const void *prev, const void *dslot, const void *sslot,
+                             bool idx32, int64_t n, uint32_t *queue, hipStream_t s)
 {
     const unsigned nb = n > 0 ? nblocks(uint64_t(n)) : 1u;
     return dispatch_prec(prec, [&](auto *sp, auto *) {
         using S = std::remove_pointer_t<decltype(sp)>;
-        hipLaunchKernelGGL((k_batch_fill<S>), dim3(nb), dim3(256), 0, s, (S *)buf, (const S *)prev, dslot, sslot, n,
-                           queue);
+        if (idx32)
+            hipLaunchKernelGGL((k_batch_fill<S, int32_t>), dim3(nb), dim3(256), 0, s, (S *)buf, (const S *)prev,
+                               (const int32_t *)dslot, (const int32_t *)sslot, n, queue);
+        else
+            hipLaunchKernelGGL((k_batch_fill<S, int64_t>), dim3(nb), dim3(256), 0, s, (S *)buf, (const S *)prev,
+                               (const int64_t *)dslot, (const int64_t *)sslot, n, queue);
         return hipGetLastError();
     });
 }

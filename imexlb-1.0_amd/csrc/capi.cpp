@@ -255,6 +255,10 @@ int imexlb_create(imexlb_ctx **out, int dim, int nx, int ny, int nz, int device,
             return fail(IMEXLB_E_ARG, "fast division self-check failed");
     k.rdy = c->g.dy;
     k.rdz = c->g.dz;
+    // XCD-contiguous block order pays for D3Q27 FP64 only (measured on
+    // MI355X); IMEXLB_XCD_REMAP=0/1 overrides
+    const char *xv = std::getenv("IMEXLB_XCD_REMAP");
+    k.xcd_remap = xv ? uint32_t(xv[0] == '1') : uint32_t(dim == 3 && precision == IMEXLB_F64);
 
     if (device >= 0) {
         int ndev = 0;

@@ -51,6 +51,7 @@ struct KParams {
     FastDiv div_px;      // / Px
     FastDiv div_py;      // / Py
     double inv_rho0;     // 1/rho0 (FP32-compute forms multiply by it)
+    uint32_t xcd_remap;  // 1: contiguous runs of blocks per XCD
 };
 
 // slot of direction q at padded cell c, host side (kernels use cslot)

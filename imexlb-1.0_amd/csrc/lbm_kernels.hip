@@ -24,19 +24,17 @@
 
 #include "kernels.hpp"
 
-// Build-time tuning of the step kernel: threads per block, streaming
-// stores for the post-collision output, XCD-contiguous block order.
+// Build-time tuning of the step kernel: threads per block, non-temporal
+// (streaming) stores for the post-collision output -- with the tiled layout
+// they save 7-14 %; with planes they cost 17 %.
 #ifndef IMEXLB_STEP_BLOCK
 #define IMEXLB_STEP_BLOCK 256
 #endif
 #ifndef IMEXLB_STEP_NT
-#define IMEXLB_STEP_NT 0
+#define IMEXLB_STEP_NT 1
 #endif
 #ifndef IMEXLB_STEP_NTLOAD
 #define IMEXLB_STEP_NTLOAD 0
-#endif
-#ifndef IMEXLB_XCD_REMAP
-#define IMEXLB_XCD_REMAP 1
 #endif
 // minimum waves per SIMD the step kernel is compiled for (0: compiler's
 // choice); it caps the registers per thread at 512/waves
@@ -141,12 +139,13 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &d)
     return uint32_t((uint64_t(__umulhi(n, d.m)) + n) >> d.s);
 }
 
-// Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous
-// run of blocks so the cache lines two neighbouring blocks share stay in
-// one L2.
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
+// Blocks are dealt round-robin to the 8 XCDs; with `remap` each XCD gets a
+// contiguous run of blocks, so the cache lines two neighbouring blocks share
+// stay in one L2 (faster for D3Q27 FP64, slower for the 16-bit forms and
+// D2Q9 on MI355X; the lattice context chooses).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb, bool remap)
 {
-    if (!IMEXLB_XCD_REMAP) return b;
+    if (!remap) return b;
     const uint32_t per = nb / 8u;
     if (b >= per * 8u) return b;
     return (b % 8u) * per + b / 8u;
@@ -481,7 +480,8 @@ __global__ IMEXLB_STEP_BOUNDS void k_step(const S *__restrict__ src, S *__restri
 {
     int x, y, z;
     uint32_t c;
-    if (!node_of<D>(kp, xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, x, y, z, c)) return;
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x, kp.xcd_remap != 0) * blockDim.x + threadIdx.x;
+    if (!node_of<D>(kp, i, x, y, z, c)) return;
     step_node<D, S, C, MODE>(src, dst, fout, pout, uxo, uyo, uzo, geo, kp, x, y, z, c);
 }
 
@@ -500,7 +500,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_step(const S *__restrict__ src,
                                                       uint32_t *__restrict__ queue)
 {
     __shared__ uint32_t next;
-    uint32_t tile = PERSIST ? 0u : xcd_block(blockIdx.x, gridDim.x);
+    uint32_t tile = PERSIST ? 0u : blockIdx.x;
     for (;;) {
         if (PERSIST) {
             __syncthreads();  // every thread has read the previous tile index
@@ -549,7 +549,8 @@ __global__ __launch_bounds__(256) void k_collide(const S *__restrict__ fin, S *_
     using L = Lat<D>;
     int x, y, z;
     uint32_t c;
-    if (!node_of<D>(kp, xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, x, y, z, c)) return;
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x, kp.xcd_remap != 0) * blockDim.x + threadIdx.x;
+    if (!node_of<D>(kp, i, x, y, z, c)) return;
     if (!geo[c]) return;
     C f[L::Q];
 #pragma unroll
@@ -568,7 +569,8 @@ __global__ __launch_bounds__(256) void k_init_pdf(S *__restrict__ fout, const do
     using L = Lat<D>;
     int x, y, z;
     uint32_t c;
-    if (!node_of<D>(kp, xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, x, y, z, c)) return;
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x, kp.xcd_remap != 0) * blockDim.x + threadIdx.x;
+    if (!node_of<D>(kp, i, x, y, z, c)) return;
     if (!geo[c]) return;
     const C pp = C(p[c]), u0 = C(ux[c]), u1 = C(uy[c]), u2 = D == 3 ? C(uz[c]) : C(0);
     C udu = u0 * u0 + u1 * u1;

@@ -73,6 +73,36 @@ def test_reference_inputs_bitwise(r_vec):
         o.close()
 
 
+def test_sphere_512x256x256_bitwise(r_vec):
+    """The benchmark lattice itself, D3Q27 sphere 513x257x257 nodes: 3 steps,
+    then f, p, u on all 33.9 M interior nodes and Monitor equal the oracle's
+    bit for bit, and the drag/lift vector agrees to 1e-12 (summation order)."""
+    lat, o, d = _run_pair("sphere512x256x256", r_vec, 3)
+    lat.advance(3)
+    o.run(0, 2, 0)
+    assert lat.monitor() == o.monitor()
+    F, Fo = lat.forces(), o.forces()
+    assert np.all(np.abs(F - Fo) <= 1e-12 * np.abs(Fo).max()), (F, Fo)
+    assert_state_equal(lat, o)
+    lat.close()
+    o.close()
+
+
+@pytest.mark.parametrize("remap", ["0", "1"])
+def test_block_order_does_not_change_results(remap, r_vec, monkeypatch):
+    """The XCD-contiguous block order is a per-lattice choice (on for D3Q27
+    FP64 by default); forced either way, 2D and 3D runs equal the oracle."""
+    monkeypatch.setenv("IMEXLB_XCD_REMAP", remap)
+    for name, steps in (("inletbody2d", 40), ("wallsphere3d", 20)):
+        lat, o, d = _run_pair(name, r_vec, steps)
+        lat.advance(steps)
+        o.run(0, steps - 1, 0)
+        assert lat.monitor() == o.monitor()
+        assert_state_equal(lat, o)
+        lat.close()
+        o.close()
+
+
 def test_graph_replay_matches_oracle(r_vec):
     """advance() replays pairs of fused steps as a HIP graph; after every
     batch, with eager Monitor and download calls in between, the state equals

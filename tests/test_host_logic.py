@@ -182,3 +182,34 @@ def test_info_sizes():
     assert info["nodes"] == 25 * 13 * 13 and info["value_bytes"] == 8
     assert info["fluid"] == len(lat.geometry_list("fluid_id"))
     lat.close()
+
+
+def test_state_comparison_catches_one_differing_value(r_vec):
+    """The bitwise checker used by the GPU parity tests: a stand-in lattice
+    that returns the oracle's own arrays passes (in small chunks too), and
+    one changed population, pressure or velocity on one interior cell
+    fails."""
+    from cases import assert_state_equal, oracle_setup
+    o, _ = oracle_setup(get_deck("tiny3d"), r_vec)
+    o.run(0, 3, 0)
+
+    class Stand:
+        def __init__(self):
+            self.f, self.p, self.u = o.f.copy(), o.p.copy(), o.u.copy()
+
+        def download_pdf(self, which, layout):
+            return self.f.copy()
+
+        def download_fields(self, layout):
+            return self.p.copy(), self.u.copy()
+
+    s = Stand()
+    assert_state_equal(s, o, chunk=97)
+    cell = int(np.nonzero(interior_mask(o))[0][1234])
+    for name, arr, idx in (("f", s.f, cell * o.nq + 5), ("p", s.p, cell), ("u", s.u, cell * o.dim + 2)):
+        keep = arr[idx]
+        arr[idx] = np.nextafter(keep, np.inf)
+        with pytest.raises(AssertionError, match=name):
+            assert_state_equal(s, o, chunk=97)
+        arr[idx] = keep
+    o.close()

@@ -96,7 +96,8 @@ struct imexlb_batch {
         const uint32_t T = uint32_t(block * cpt);
         std::vector<uint32_t> tc, tf;
         for (int i = 0; i < ncase; ++i)
-            for (uint32_t first = 0; first < cases[size_t(i)].kp.N; first += T) {
+            for (uint32_t first = 0; first < (IMEXLB_CELLMAP ? cases[size_t(i)].kp.ncell : cases[size_t(i)].kp.N);
+                 first += T) {
                 tc.push_back(uint32_t(i));
                 tf.push_back(first);
             }

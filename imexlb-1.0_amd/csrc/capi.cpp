@@ -569,22 +569,24 @@ static int one_step(imexlb_ctx *c)
     return IMEXLB_OK;
 }
 
-// Two fused steps from the steady state (stage S_FUSED, cur == X) as one HIP
-// graph: fill(1-X) pull(1-X -> X) fill(X) pull(X -> 1-X).  After it the
-// state is again S_FUSED with cur == X, so the graph replays.  Only kernels
-// are recorded (no memset or copy nodes).
-static int pair_graph(imexlb_ctx *c, hipGraphExec_t *out)
+// Pairs of fused steps from the steady state (stage S_FUSED, cur == X) as
+// one HIP graph: per pair fill(1-X) pull(1-X -> X) fill(X) pull(X -> 1-X).
+// After it the state is again S_FUSED with cur == X, so the graph replays.
+// Only kernels are recorded (no memset or copy nodes).  pairs: 1, or
+// kLongPairs (fewer launches for small lattices).
+static int step_graph(imexlb_ctx *c, int pairs, hipGraphExec_t *out)
 {
     const int X = c->cur;
-    if (c->graph[X]) {
-        *out = c->graph[X];
+    hipGraphExec_t &slot = pairs == 1 ? c->graph[X] : c->graph_long[X];
+    if (slot) {
+        *out = slot;
         return IMEXLB_OK;
     }
     StepArgs a = c->args();
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
-    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
-        const int b = k == 0 ? 1 - X : X;
+    for (int k = 0; k < 2 * pairs && e == hipSuccess; ++k) {
+        const int b = k % 2 == 0 ? 1 - X : X;
         e = launch_fill(c->prec, c->fb[b], c->fb[1 - b], c->fill_dst, c->fill_src, c->fill32, c->nfill, c->stream);
         if (e != hipSuccess) break;
         a.src = c->fb[b];
@@ -602,7 +604,7 @@ static int pair_graph(imexlb_ctx *c, hipGraphExec_t *out)
     e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(IMEXLB_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-    c->graph[X] = x;
+    slot = x;
     *out = x;
     return IMEXLB_OK;
 }
@@ -618,10 +620,19 @@ int imexlb_advance(imexlb_ctx *c, int64_t nsteps)
             int rc = one_step(c);
             if (rc) return rc;
         }
+        const int64_t longs = (nsteps - i) / (2 * imexlb_ctx::kLongPairs);
+        if (longs > 0) {
+            hipGraphExec_t g;
+            int rc = step_graph(c, imexlb_ctx::kLongPairs, &g);
+            if (rc) return rc;
+            for (int64_t k = 0; k < longs; ++k) HIP_TRY(hipGraphLaunch(g, c->stream));
+            i += 2 * imexlb_ctx::kLongPairs * longs;
+            c->have_f = c->have_pu = false;
+        }
         const int64_t pairs = (nsteps - i) / 2;
         if (pairs > 0) {
             hipGraphExec_t g;
-            int rc = pair_graph(c, &g);
+            int rc = step_graph(c, 1, &g);
             if (rc) return rc;
             for (int64_t k = 0; k < pairs; ++k) HIP_TRY(hipGraphLaunch(g, c->stream));
             i += 2 * pairs;

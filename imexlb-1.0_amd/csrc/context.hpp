@@ -85,14 +85,17 @@ struct imexlb_ctx {
 
     // replay of two fused steps as a HIP graph, one per starting buffer
     bool graphs = true;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph[2] = {nullptr, nullptr};       // one pair of fused steps, by cur
+    hipGraphExec_t graph_long[2] = {nullptr, nullptr};  // kLongPairs pairs, by cur
+    static constexpr int kLongPairs = 16;
     void drop_graphs()
     {
-        for (auto &g : graph)
-            if (g) {
-                (void)hipGraphExecDestroy(g);
-                g = nullptr;
-            }
+        for (auto *set : {graph, graph_long})
+            for (int k = 0; k < 2; ++k)
+                if (set[k]) {
+                    (void)hipGraphExecDestroy(set[k]);
+                    set[k] = nullptr;
+                }
     }
 
     // profiling

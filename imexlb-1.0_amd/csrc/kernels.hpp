@@ -35,6 +35,12 @@ struct FastDiv {
 #endif
 static_assert(IMEXLB_TILE >= 0 && (IMEXLB_TILE & (IMEXLB_TILE - 1)) == 0, "tile size: 0 or a power of two");
 
+// Thread i of the step kernels handles padded cell i (1; threads on ghost
+// cells leave, and a wave covers one tile) or interior node i (0).
+#ifndef IMEXLB_CELLMAP
+#define IMEXLB_CELLMAP 1
+#endif
+
 // Geometry and scalars every kernel needs (passed by value).
 struct KParams {
     int nx, ny, nz;      // largest interior node index per axis

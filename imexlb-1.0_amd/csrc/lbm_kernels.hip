@@ -41,10 +41,6 @@
 #ifndef IMEXLB_STEP_WAVES
 #define IMEXLB_STEP_WAVES 0
 #endif
-// thread i handles padded cell i (1) or interior node i (0)
-#ifndef IMEXLB_CELLMAP
-#define IMEXLB_CELLMAP 1
-#endif
 
 namespace imexlb {
 
@@ -516,7 +512,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_step(const S *__restrict__ src,
             const uint32_t i = first + uint32_t(k * BLOCK) + threadIdx.x;
             int x, y, z;
             uint32_t c;
-            if (decode<D>(bc.kp, i, x, y, z, c))
+            if (node_of<D>(bc.kp, i, x, y, z, c))
                 step_node<D, S, C, K_COLLIDE>(src + bc.pop_base, dst + bc.pop_base, nullptr, nullptr, nullptr, nullptr,
                                               nullptr, geo + bc.geo_base, bc.kp, x, y, z, c);
         }

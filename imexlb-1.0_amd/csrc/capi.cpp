@@ -108,7 +108,7 @@ int build_device_tables(const imexlb_ctx *c, DeviceTables &t)
     for (int32_t id : g.fluid_id) {
         int x, y, z;
         g.coords(id, x, y, z);
-        t.geo[size_t(c->pidx(x, y, z))] = 1;
+        t.geo[size_t(c->pidx(x, y, z))] = G_FLUID;
     }
     if (!g.fluid_id.empty()) {
         int x, y, z;
@@ -149,6 +149,43 @@ int build_device_tables(const imexlb_ctx *c, DeviceTables &t)
         }
         t.fill_dst.swap(d);
         t.fill_src.swap(s);
+    }
+    // the same table grouped by source node (padded cell, direction) for the
+    // fused step's push copies; zero entries are left out (never written,
+    // the buffers start at 0)
+    {
+        struct Copy {
+            uint32_t cell;
+            uint8_t q;
+            int64_t dst;
+        };
+        std::vector<Copy> v;
+        v.reserve(c->bc.entries.size());
+        t.push_ok = true;
+        for (const FillEntry &e : c->bc.entries) {
+            if (e.ref_src == -1) continue;
+            const bool prev = e.ref_src <= -2;
+            const int64_t rs = prev ? decode_prev(e.ref_src) : e.ref_src;
+            int x, y, z;
+            g.coords(rs / c->Q, x, y, z);
+            const uint32_t cell = uint32_t(c->pidx(x, y, z));
+            // this step's value comes from a fluid node, a previous one from a body cell
+            if (bool(t.geo[cell] & G_FLUID) == prev) t.push_ok = false;
+            v.push_back({cell, uint8_t(rs % c->Q), dev_slot(e.ref_slot)});
+        }
+        std::sort(v.begin(), v.end(), [](const Copy &a, const Copy &b) {
+            return a.cell != b.cell ? a.cell < b.cell : a.q != b.q ? a.q < b.q : a.dst < b.dst;
+        });
+        t.push_beg.assign(size_t(c->kp.ncell) + 1, 0);
+        t.push_q.resize(v.size());
+        t.push_dst.resize(v.size());
+        for (size_t k = 0; k < v.size(); ++k) {
+            ++t.push_beg[size_t(v[k].cell) + 1];
+            t.push_q[k] = v[k].q;
+            t.push_dst[k] = v[k].dst;
+            t.geo[v[k].cell] |= G_PUSH;
+        }
+        for (size_t k = 1; k < t.push_beg.size(); ++k) t.push_beg[k] += t.push_beg[k - 1];
     }
     // body links: fluid node x, direction q, interior solid neighbour x+e_q
     t.link_out.clear();
@@ -268,6 +305,8 @@ int imexlb_create(imexlb_ctx **out, int dim, int nx, int ny, int nz, int device,
         c->host_only = false;
         const char *gv = std::getenv("IMEXLB_GRAPHS");
         c->graphs = !(gv && gv[0] == '0');
+        const char *pv = std::getenv("IMEXLB_PUSH");
+        c->push_enabled = !(pv && pv[0] == '0');
         HIP_TRY(hipSetDevice(device));
         HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         const size_t pop = size_t(c->Q) * size_t(c->plane) * c->ssz;
@@ -332,11 +371,20 @@ int imexlb_set_geometry(imexlb_ctx *c, double pos, double radius)
     c->skip_c = t.skip_c;
     HIP_TRY(hipMemcpy(c->geo, t.geo.data(), t.geo.size(), hipMemcpyHostToDevice));
     for (void *q : {(void *)c->fill_dst, (void *)c->fill_src, (void *)c->link_out, (void *)c->link_back,
-                    (void *)c->link_q, (void *)c->force_partial, (void *)c->force_out})
+                    (void *)c->link_q, (void *)c->force_partial, (void *)c->force_out, (void *)c->push_beg,
+                    (void *)c->push_q, c->push_dst})
         if (q) (void)hipFree(q);
     c->fill_dst = c->fill_src = c->link_out = c->link_back = nullptr;
     c->link_q = nullptr;
     c->force_partial = c->force_out = nullptr;
+    c->push_beg = nullptr;
+    c->push_q = nullptr;
+    c->push_dst = nullptr;
+    // boundary slots whose value is 0 are never written by the pushed copies:
+    // start both buffers from 0 for this geometry
+    const size_t pop = size_t(c->Q) * size_t(c->plane) * c->ssz;
+    HIP_TRY(hipMemset(c->fb[0], 0, pop));
+    HIP_TRY(hipMemset(c->fb[1], 0, pop));
     c->nfill = int64_t(t.fill_dst.size());
     c->nlink = int64_t(t.link_out.size());
     const size_t nf = std::max<size_t>(t.fill_dst.size(), 1), nl = std::max<size_t>(t.link_out.size(), 1);
@@ -351,6 +399,14 @@ int imexlb_set_geometry(imexlb_ctx *c, double pos, double radius)
         return rc;
     HIP_TRY(upload_slots(c->fill_dst, t.fill_dst, c->fill32));
     HIP_TRY(upload_slots(c->fill_src, t.fill_src, c->fill32));
+    c->push_on = c->push_enabled && t.push_ok;
+    const size_t np = std::max<size_t>(t.push_q.size(), 1);
+    if ((rc = dev_alloc(c, (void **)&c->push_beg, t.push_beg.size() * sizeof(int32_t))) ||
+        (rc = dev_alloc(c, (void **)&c->push_q, np)) || (rc = dev_alloc(c, &c->push_dst, np * isz)))
+        return rc;
+    HIP_TRY(hipMemcpy(c->push_beg, t.push_beg.data(), t.push_beg.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!t.push_q.empty()) HIP_TRY(hipMemcpy(c->push_q, t.push_q.data(), t.push_q.size(), hipMemcpyHostToDevice));
+    HIP_TRY(upload_slots(c->push_dst, t.push_dst, c->fill32));
     if (c->nlink) {
         HIP_TRY(hipMemcpy(c->link_out, t.link_out.data(), t.link_out.size() * 8, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(c->link_back, t.link_back.data(), t.link_back.size() * 8, hipMemcpyHostToDevice));
@@ -490,6 +546,7 @@ int imexlb_collision(imexlb_ctx *c)
     if (c->stage == S_FUSED) {  // the fused kernel already relaxed this step
         c->cur = 1 - c->cur;
         c->stage = S_COLLIDED;
+        c->prefilled = c->push_on;  // and wrote the boundary copies of its output
         return IMEXLB_OK;
     }
     if (c->stage != S_PRE) return fail(IMEXLB_E_STATE, "Collision needs InitPDF or a completed step");
@@ -503,6 +560,7 @@ int imexlb_collision(imexlb_ctx *c)
     c->cur = 1 - c->cur;
     c->have_fb = true;
     c->stage = S_COLLIDED;
+    c->prefilled = false;
     return IMEXLB_OK;
 }
 
@@ -510,8 +568,11 @@ int imexlb_boundary_condition(imexlb_ctx *c)
 {
     CHECK_DEV(c);
     if (c->stage != S_COLLIDED) return fail(IMEXLB_E_STATE, "BoundaryCondition must follow Collision");
-    int rc = c->run_fill(c->cur);
-    if (rc) return rc;
+    if (!c->prefilled) {
+        int rc = c->run_fill(c->cur);
+        if (rc) return rc;
+    }
+    c->prefilled = false;
     c->stage = S_FILLED;
     return IMEXLB_OK;
 }
@@ -536,7 +597,7 @@ int imexlb_post_processing(imexlb_ctx *c)
 {
     CHECK_DEV(c);
     if (c->stage == S_PENDING) {
-        int rc = c->run_step(K_COLLIDE, c->fb[c->cur], c->fb[1 - c->cur], nullptr);
+        int rc = c->run_step(c->collide_mode(), c->fb[c->cur], c->fb[1 - c->cur], nullptr);
         if (rc) return rc;
         c->lazy_src = c->cur;
         c->have_f = c->have_pu = false;
@@ -570,8 +631,9 @@ static int one_step(imexlb_ctx *c)
 }
 
 // Pairs of fused steps from the steady state (stage S_FUSED, cur == X) as
-// one HIP graph: per pair fill(1-X) pull(1-X -> X) fill(X) pull(X -> 1-X).
-// After it the state is again S_FUSED with cur == X, so the graph replays.
+// one HIP graph: per pair fill(1-X) pull(1-X -> X) fill(X) pull(X -> 1-X),
+// or without the fills when the pulls push their boundary copies.  After it
+// the state is again S_FUSED with cur == X, so the graph replays.
 // Only kernels are recorded (no memset or copy nodes).  pairs: 1, or
 // kLongPairs (fewer launches for small lattices).
 static int step_graph(imexlb_ctx *c, int pairs, hipGraphExec_t *out)
@@ -587,12 +649,15 @@ static int step_graph(imexlb_ctx *c, int pairs, hipGraphExec_t *out)
     hipError_t e = hipSuccess;
     for (int k = 0; k < 2 * pairs && e == hipSuccess; ++k) {
         const int b = k % 2 == 0 ? 1 - X : X;
-        e = launch_fill(c->prec, c->fb[b], c->fb[1 - b], c->fill_dst, c->fill_src, c->fill32, c->nfill, c->stream);
-        if (e != hipSuccess) break;
+        if (!c->push_on) {  // with pushed copies every buffer is complete already
+            e = launch_fill(c->prec, c->fb[b], c->fb[1 - b], c->fill_dst, c->fill_src, c->fill32, c->nfill,
+                            c->stream);
+            if (e != hipSuccess) break;
+        }
         a.src = c->fb[b];
         a.dst = c->fb[1 - b];
         a.fout = nullptr;
-        e = launch_step(c->dim, c->prec, K_COLLIDE, a, c->stream);
+        e = launch_step(c->dim, c->prec, c->collide_mode(), a, c->stream);
     }
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(c->stream, &g);

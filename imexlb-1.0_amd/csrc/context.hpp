@@ -67,6 +67,13 @@ struct imexlb_ctx {
     uint8_t *geo = nullptr;
     void *fill_dst = nullptr, *fill_src = nullptr;  // int32 (fill32) or int64 slots
     bool fill32 = false;
+    // boundary copies pushed by the fused step (grouped resolved table)
+    int32_t *push_beg = nullptr;
+    uint8_t *push_q = nullptr;
+    void *push_dst = nullptr;   // int32 (fill32) or int64 slots
+    bool push_enabled = true;   // IMEXLB_PUSH=0 keeps the separate fill
+    bool push_on = false;       // enabled and the table qualifies
+    bool prefilled = false;     // buffer cur already holds its boundary copies
     int64_t nfill = 0;
     unsigned long long *red = nullptr;
     // body links for the momentum-exchange force
@@ -116,6 +123,9 @@ struct imexlb_ctx {
         a.uy = uy;
         a.uz = uz;
         a.geo = geo;
+        a.push.beg = push_beg;
+        a.push.q = push_q;
+        a.push.dst = push_dst;
         return a;
     }
 
@@ -180,13 +190,16 @@ struct imexlb_ctx {
         });
     }
 
+    // mode bits of the fused collide step for this lattice
+    int collide_mode() const { return push_on ? (K_COLLIDE | K_PUSH | (fill32 ? 0 : K_IDX64)) : K_COLLIDE; }
+
     int run_step(int mode, const void *src, void *dst, void *fout)
     {
         StepArgs a = args();
         a.src = src;
         a.dst = dst;
         a.fout = fout;
-        return launch(mode == K_COLLIDE ? 0 : 2, [&] { return launch_step(dim, prec, mode, a, stream); });
+        return launch((mode & K_COLLIDE) ? 0 : 2, [&] { return launch_step(dim, prec, mode, a, stream); });
     }
 
     // rebuild f, p, u of the current pre-collision state if the fused
@@ -215,11 +228,15 @@ struct imexlb_ctx {
             (void)hipEventDestroy(t.b);
         }
         timed.clear();
-        void *ptrs[] = {fb[0], fb[1], f,        p,         ux,     uy,           uz,       rv,
-                        geo,   fill_dst, fill_src, red, link_out, link_back, link_q, force_partial, force_out};
+        void *ptrs[] = {fb[0],     fb[1],  f,         p,        ux,       uy,        uz,     rv,
+                        geo,       fill_dst, fill_src, red,     link_out, link_back, link_q, force_partial,
+                        force_out, push_beg, push_q,   push_dst};
         for (void *q : ptrs)
             if (q) (void)hipFree(q);
         fb[0] = fb[1] = f = nullptr;
+        push_beg = nullptr;
+        push_q = nullptr;
+        push_dst = nullptr;
         p = ux = uy = uz = rv = nullptr;
         geo = nullptr;
         fill_dst = fill_src = nullptr;
@@ -255,6 +272,10 @@ struct DeviceTables {
     std::vector<uint8_t> geo;
     uint32_t skip_c = 0xffffffffu;
     std::vector<int64_t> fill_dst, fill_src;
+    std::vector<int32_t> push_beg;  // ncell + 1 offsets
+    std::vector<uint8_t> push_q;
+    std::vector<int64_t> push_dst;
+    bool push_ok = true;  // every copy's source is a fluid node or a body cell
     std::vector<int64_t> link_out, link_back;
     std::vector<int8_t> link_q;
 };

@@ -72,7 +72,23 @@ inline int64_t pop_slot(int64_t q, int64_t c, int64_t plane, int Q)
 #endif
 }
 
-enum : int { K_COLLIDE = 1, K_WRITE_F = 2, K_WRITE_PU = 4, K_NOSHIFT = 8 };
+// K_PUSH: the collide step also writes the boundary copies of its output
+// (PushTable), so the buffer needs no boundary fill; K_IDX64: the push
+// table's destination slots are int64 (else int32).
+enum : int { K_COLLIDE = 1, K_WRITE_F = 2, K_WRITE_PU = 4, K_NOSHIFT = 8, K_PUSH = 16, K_IDX64 = 32 };
+
+// node-type byte on the device: fluid node; node with boundary copies to push
+enum : uint8_t { G_FLUID = 1, G_PUSH = 2 };
+
+// The resolved boundary table grouped by source node: entries beg[c] ..
+// beg[c+1]-1 of padded cell c copy its direction q[e] to slot dst[e] of the
+// same buffer -- the post-collision value of a fluid node, or for a body
+// cell the previous step's value (from the other buffer).
+struct PushTable {
+    const int32_t *beg = nullptr;
+    const uint8_t *q = nullptr;
+    const void *dst = nullptr;  // int32 or int64 slots (K_IDX64)
+};
 
 // One case of a batched ensemble: its parameters and where its populations
 // and node types start in the shared arrays.
@@ -89,6 +105,7 @@ struct StepArgs {
     void *fout = nullptr;       // array f output
     double *p = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr;
     const uint8_t *geo = nullptr;
+    PushTable push;
 };
 
 // prec: 0 FP64, 1 FP32, 2 FP16 storage + FP32 arithmetic, 3 BF16 storage + FP32 arithmetic

@@ -406,18 +406,26 @@ __device__ __forceinline__ constexpr int load_order(int j)
 // COLLIDE alone is the fused Propagation+PostProcessing+Collision pull;
 // WRITE_F alone is Propagation; NOSHIFT|WRITE_PU is PostProcessing;
 // WRITE_F|WRITE_PU materialises f, p, u lazily.
+template <int MODE>
+__device__ __forceinline__ uint64_t push_slot(const PushTable &pt, int32_t e)
+{
+    return (MODE & K_IDX64) ? uint64_t(static_cast<const int64_t *>(pt.dst)[e])
+                            : uint64_t(static_cast<const int32_t *>(pt.dst)[e]);
+}
+
 template <int D, class S, class C, int MODE>
 __device__ __forceinline__ void step_node(const S *__restrict__ src, S *__restrict__ dst, S *__restrict__ fout,
                                           double *__restrict__ pout, double *__restrict__ uxo,
                                           double *__restrict__ uyo, double *__restrict__ uzo,
                                           const uint8_t *__restrict__ geo, const KParams &kp, int x, int y, int z,
-                                          uint32_t c)
+                                          uint32_t c, const PushTable &pt = PushTable{})
 {
     using L = Lat<D>;
     // The node's type is loaded together with its populations, not before
     // them, so the gather does not wait for it (every gathered slot is in
     // the arrays also for solid nodes).
-    const uint8_t fluid = geo[c];
+    const uint8_t g = geo[c];
+    const bool fluid = (g & G_FLUID) != 0;
     C f[L::Q];
 #pragma unroll
     for (int j = 0; j < L::Q; ++j) {
@@ -434,6 +442,10 @@ __device__ __forceinline__ void step_node(const S *__restrict__ src, S *__restri
                 uyo[c] = 0.0;
             }
             if (x == kp.nx) pout[c] = 0.0;
+        }
+        if ((MODE & K_PUSH) && (g & G_PUSH)) {  // a body cell: its previous-step values
+            const int32_t e1 = pt.beg[c + 1];
+            for (int32_t e = pt.beg[c]; e < e1; ++e) dst[push_slot<MODE>(pt, e)] = src[cslot<L::Q>(kp, pt.q[e], c)];
         }
         return;
     }
@@ -458,6 +470,10 @@ __device__ __forceinline__ void step_node(const S *__restrict__ src, S *__restri
             else
                 *o = Store<S>::template st<C>(v);
         });
+        if ((MODE & K_PUSH) && (g & G_PUSH)) {  // boundary copies of this node's output (read back)
+            const int32_t e1 = pt.beg[c + 1];
+            for (int32_t e = pt.beg[c]; e < e1; ++e) dst[push_slot<MODE>(pt, e)] = dst[cslot<L::Q>(kp, pt.q[e], c)];
+        }
     }
 }
 
@@ -472,13 +488,13 @@ __global__ IMEXLB_STEP_BOUNDS void k_step(const S *__restrict__ src, S *__restri
                                               S *__restrict__ fout, double *__restrict__ pout,
                                               double *__restrict__ uxo, double *__restrict__ uyo,
                                               double *__restrict__ uzo, const uint8_t *__restrict__ geo,
-                                              KParams kp)
+                                              KParams kp, PushTable pt)
 {
     int x, y, z;
     uint32_t c;
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x, kp.xcd_remap != 0) * blockDim.x + threadIdx.x;
     if (!node_of<D>(kp, i, x, y, z, c)) return;
-    step_node<D, S, C, MODE>(src, dst, fout, pout, uxo, uyo, uzo, geo, kp, x, y, z, c);
+    step_node<D, S, C, MODE>(src, dst, fout, pout, uxo, uyo, uzo, geo, kp, x, y, z, c, pt);
 }
 
 // ---------------------------------------------------------------------------
@@ -547,7 +563,7 @@ __global__ __launch_bounds__(256) void k_collide(const S *__restrict__ fin, S *_
     uint32_t c;
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x, kp.xcd_remap != 0) * blockDim.x + threadIdx.x;
     if (!node_of<D>(kp, i, x, y, z, c)) return;
-    if (!geo[c]) return;
+    if (!(geo[c] & G_FLUID)) return;
     C f[L::Q];
 #pragma unroll
     for (int q = 0; q < L::Q; ++q) f[q] = Store<S>::ld(fin[cslot<L::Q>(kp, q, c)]);
@@ -567,7 +583,7 @@ __global__ __launch_bounds__(256) void k_init_pdf(S *__restrict__ fout, const do
     uint32_t c;
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x, kp.xcd_remap != 0) * blockDim.x + threadIdx.x;
     if (!node_of<D>(kp, i, x, y, z, c)) return;
-    if (!geo[c]) return;
+    if (!(geo[c] & G_FLUID)) return;
     const C pp = C(p[c]), u0 = C(ux[c]), u1 = C(uy[c]), u2 = D == 3 ? C(uz[c]) : C(0);
     C udu = u0 * u0 + u1 * u1;
     if (D == 3) udu = udu + u2 * u2;
@@ -606,7 +622,7 @@ __global__ __launch_bounds__(256) void k_init_up(double *__restrict__ p, double 
     if (!decode<D>(kp, blockIdx.x * blockDim.x + threadIdx.x, x, y, z, c)) return;
     const int64_t rid = int64_t(x + 3) + kp.rdy * (y + 3) + (D == 3 ? kp.rdz * (z + 3) : 0);
     const double r = 1.0 + (rv[rid % 100] - 0.5) * 2.0 * 0.2;
-    const double g = double(geo[c]);
+    const double g = double(geo[c] & G_FLUID);
     p[c] = 0.0;
     if (D == 2) {
         const double yy = double(y);
@@ -661,7 +677,7 @@ __global__ __launch_bounds__(256) void k_monitor(const double *__restrict__ ux, 
             int x, y, z;
             uint32_t c;
             a[k] = 0.0;
-            if (i < kp.N && decode<D>(kp, uint32_t(i), x, y, z, c) && geo[c] && c != skip)
+            if (i < kp.N && decode<D>(kp, uint32_t(i), x, y, z, c) && (geo[c] & G_FLUID) && c != skip)
                 a[k] = ux[c] * ux[c] + uy[c] * uy[c];
         }
 #pragma unroll
@@ -749,7 +765,7 @@ static hipError_t step_dm(int prec, const StepArgs &a, hipStream_t s)
         using C = std::remove_pointer_t<decltype(cp)>;
         const unsigned nb = unsigned((node_threads(a.kp) + IMEXLB_STEP_BLOCK - 1) / IMEXLB_STEP_BLOCK);
         hipLaunchKernelGGL((k_step<D, S, C, MODE>), dim3(nb), dim3(IMEXLB_STEP_BLOCK), 0, s, (const S *)a.src,
-                           (S *)a.dst, (S *)a.fout, a.p, a.ux, a.uy, a.uz, a.geo, a.kp);
+                           (S *)a.dst, (S *)a.fout, a.p, a.ux, a.uy, a.uz, a.geo, a.kp, a.push);
         return hipGetLastError();
     });
 }
@@ -759,6 +775,8 @@ static hipError_t step_d(int prec, int mode, const StepArgs &a, hipStream_t s)
 {
     switch (mode) {
     case K_COLLIDE: return step_dm<D, K_COLLIDE>(prec, a, s);
+    case K_COLLIDE | K_PUSH: return step_dm<D, K_COLLIDE | K_PUSH>(prec, a, s);
+    case K_COLLIDE | K_PUSH | K_IDX64: return step_dm<D, K_COLLIDE | K_PUSH | K_IDX64>(prec, a, s);
     case K_WRITE_F: return step_dm<D, K_WRITE_F>(prec, a, s);
     case K_WRITE_F | K_WRITE_PU: return step_dm<D, K_WRITE_F | K_WRITE_PU>(prec, a, s);
     case K_NOSHIFT | K_WRITE_PU: return step_dm<D, K_NOSHIFT | K_WRITE_PU>(prec, a, s);

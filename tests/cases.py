@@ -15,6 +15,14 @@ TINY = {
     "inletbody2d": (2, "40 20\n0.5 0.0 20\n1.5 1 1\n0.02 0.001\n"),
     # sphere touching the z walls: radius 6 centred at (8, 6, 6) in 25x13x13
     "wallsphere3d": (3, "24 12 12\n0.5 0.333333333333333 20\n1.5 1 1\n0.02 0.001\n"),
+    # sphere wider than the periodic y extent: radius 6 at (15, 4, 10) in
+    # 31x9x21, so the body crosses both periodic y faces
+    "ysphere3d": (3, "30 8 20\n0.4 0.5 20\n1.6 1 1\n0.02 0.001\n"),
+    # sphere cut by the outlet plane: radius 3 centred at (24, 6, 6) in 25x13x13
+    "outletsphere3d": (3, "24 12 12\n0.25 1.0 20\n1.6 1 1\n0.02 0.001\n"),
+    # odd sizes, body off the half-integer centre line: 38x24 and 24x18x14
+    "ragged2d": (2, "37 23\n0.2 0.3 20\n1.7 1 1\n0.02 0.001\n"),
+    "ragged3d": (3, "23 17 13\n0.3 0.35 20\n1.55 1 1\n0.02 0.001\n"),
 }
 
 

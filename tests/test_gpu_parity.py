@@ -39,7 +39,8 @@ def _advance_with_monitor(lat, o, d, steps):
 
 
 @pytest.mark.parametrize("name,steps", [("tiny2d", 70), ("tiny3d", 44), ("inletbody2d", 145),
-                                        ("wallsphere3d", 52)])
+                                        ("wallsphere3d", 52), ("ysphere3d", 40), ("outletsphere3d", 40),
+                                        ("ragged2d", 90), ("ragged3d", 40)])
 @pytest.mark.parametrize("fused", [True, False])
 def test_full_run_bitwise_small(name, steps, fused, r_vec):
     lat, o, d = _run_pair(name, r_vec, steps, fused)

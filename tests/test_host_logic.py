@@ -56,7 +56,8 @@ def _host_ctx(deck):
     return lat, o
 
 
-@pytest.mark.parametrize("name", ["tiny2d", "tiny3d", "inletbody2d", "wallsphere3d", "cyl400x100", "input.in"])
+@pytest.mark.parametrize("name", ["tiny2d", "tiny3d", "inletbody2d", "wallsphere3d", "ysphere3d", "outletsphere3d",
+                                  "ragged2d", "ragged3d", "cyl400x100", "input.in"])
 def test_geometry_matches_oracle(name):
     lat, o = _host_ctx(get_deck(name))
     assert np.array_equal(lat.geometry_geo(), o.geo)
@@ -133,7 +134,8 @@ def test_table_driven_steps_reproduce_oracle_populations(r_vec):
         o.close()
 
 
-@pytest.mark.parametrize("name", ["tiny2d", "tiny3d", "inletbody2d", "wallsphere3d", "cyl400x100", "input.in"])
+@pytest.mark.parametrize("name", ["tiny2d", "tiny3d", "inletbody2d", "wallsphere3d", "ysphere3d", "outletsphere3d",
+                                  "ragged2d", "ragged3d", "cyl400x100", "input.in"])
 def test_bc_table_matches_oracle_boundary_condition(name):
     lat, o = _host_ctx(get_deck(name))
     stats = _check_bc_table(lat, o)

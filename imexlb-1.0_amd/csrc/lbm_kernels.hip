@@ -7,15 +7,17 @@
 // planes, slot = q*plane + c).
 // Interior fluid nodes hold post-collision values; interior solid nodes
 // hold 0 (the reference never writes them, lbm.f90:57); the ghost layer and
-// the body cells the reference rewrites (its b_user list) are filled once
-// per step from a resolved table (k_fill), which reproduces PassF plus the
-// five copy phases of BoundaryCondition (lbm.f90:96-155) slot for slot.
+// the body cells the reference rewrites (its b_user list) take their values
+// from a resolved table that reproduces PassF plus the five copy phases of
+// BoundaryCondition (lbm.f90:96-155) slot for slot -- either as its own fill
+// kernel (k_fill) or written by the fused step that produces the buffer
+// (K_PUSH, the table grouped by source node).
 //
 // One thread per padded cell (threads on ghost cells leave), x fastest, so a
-// wave covers one tile.  In FP64 form every kernel keeps
-// the reference's operation order (lbm.f90:56-74, 163-199) and the library
-// is compiled without floating-point contraction, so results are the
-// reference's IEEE-754 results.
+// wave covers one tile.  In FP64 form every kernel keeps the reference's
+// operation order (lbm.f90:56-74, 163-199) and the library is compiled
+// without floating-point contraction, so results are the reference's
+// IEEE-754 results.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,9 +26,10 @@
 
 #include "kernels.hpp"
 
-// Build-time tuning of the step kernel: threads per block, non-temporal
-// (streaming) stores for the post-collision output -- with the tiled layout
-// they save 7-14 %; with planes they cost 17 %.
+// Build-time tuning of the step kernel (defaults measured best on MI355X):
+// threads per block; non-temporal (streaming) stores for the post-collision
+// output -- with the tiled layout they save 7-14 %, with planes they cost
+// 17 %; non-temporal loads (cost 8 %).
 #ifndef IMEXLB_STEP_BLOCK
 #define IMEXLB_STEP_BLOCK 256
 #endif

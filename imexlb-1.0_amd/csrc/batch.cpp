@@ -198,6 +198,12 @@ extern "C" {
 int imexlb_batch_create(imexlb_batch **out, int ncase, const int32_t *nxny, const double *params, int device,
                         int precision)
 {
+    return imexlb_batch_create_shapes(out, ncase, nxny, params, nullptr, device, precision);
+}
+
+int imexlb_batch_create_shapes(imexlb_batch **out, int ncase, const int32_t *nxny, const double *params,
+                               const double *shapes, int device, int precision)
+{
     if (!out || !nxny || !params || ncase <= 0) return fail(IMEXLB_E_ARG, "batch arguments");
     *out = nullptr;
     if (precision < IMEXLB_F64 || precision > IMEXLB_F16_F32)
@@ -227,7 +233,9 @@ int imexlb_batch_create(imexlb_batch **out, int ncase, const int32_t *nxny, cons
         b->host.push_back(c);
         c->ssz = b->ssz;
         const double *pr = params + 5 * i;  // pos, radius, ome, rho0, uu
-        if ((rc = imexlb_set_params(c, pr[2], pr[3], pr[4])) || (rc = imexlb_set_geometry(c, pr[0], pr[1])))
+        const double aspect = shapes ? shapes[2 * i] : 1.0, angle = shapes ? shapes[2 * i + 1] : 0.0;
+        if ((rc = imexlb_set_params(c, pr[2], pr[3], pr[4])) ||
+            (rc = imexlb_set_geometry_shape(c, pr[0], pr[1], aspect, angle)))
             return rc;
         DeviceTables t;
         if ((rc = build_device_tables(c, t))) return rc;
@@ -302,17 +310,16 @@ int imexlb_batch_destroy(imexlb_batch *b)
     return IMEXLB_OK;
 }
 
-int imexlb_batch_init(imexlb_batch *b, const double *r_vec)
+// InitUP (when r_vec is given; otherwise p, u hold uploaded fields), InitPDF
+// and the first Collision of every case
+static int batch_start(imexlb_batch *b, const double *r_vec)
 {
-    CHECK_BATCH(b);
-    if (!r_vec) return fail(IMEXLB_E_ARG, "null r_vec");
-    HIP_TRY(hipMemcpyAsync(b->rv, r_vec, 100 * sizeof(double), hipMemcpyHostToDevice, b->stream));
     // both buffers start at zero: the body cells' previous values
     HIP_TRY(hipMemsetAsync(b->fb[0], 0, size_t(b->pop_total) * b->ssz, b->stream));
     HIP_TRY(hipMemsetAsync(b->fb[1], 0, size_t(b->pop_total) * b->ssz, b->stream));
     for (int i = 0; i < b->ncase; ++i) {
         StepArgs a = b->case_args(i);
-        hipError_t e = launch_init_up(2, a, b->rv, b->stream);
+        hipError_t e = r_vec ? launch_init_up(2, a, b->rv, b->stream) : hipSuccess;
         a.fout = b->pop(b->f, i);
         if (e == hipSuccess) e = launch_init_pdf(2, b->prec, a, b->stream);
         a.src = b->pop(b->f, i);
@@ -326,6 +333,39 @@ int imexlb_batch_init(imexlb_batch *b, const double *r_vec)
     b->have_pu = true;
     b->initialized = true;
     return IMEXLB_OK;
+}
+
+int imexlb_batch_init(imexlb_batch *b, const double *r_vec)
+{
+    CHECK_BATCH(b);
+    if (!r_vec) return fail(IMEXLB_E_ARG, "null r_vec");
+    HIP_TRY(hipMemcpyAsync(b->rv, r_vec, 100 * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    return batch_start(b, r_vec);
+}
+
+int imexlb_batch_init_fields(imexlb_batch *b, const double *p, const double *u)
+{
+    CHECK_BATCH(b);
+    if (!p || !u) return fail(IMEXLB_E_ARG, "null fields");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    int64_t off = 0;  // start of case i in the concatenated interior fields
+    for (int i = 0; i < b->ncase; ++i) {
+        const imexlb_ctx *c = b->host[size_t(i)];
+        const int64_t N = c->kp.N;
+        const size_t n = size_t(c->plane);
+        double *dsts[3] = {b->p, b->ux, b->uy};
+        for (int k = 0; k < 3; ++k) {
+            std::vector<double> h(n, 0.0);
+            const double *src = k == 0 ? p + off : u + 2 * off + int64_t(k - 1) * N;
+            int64_t m = 0;
+            for (int y = 0; y <= c->kp.ny; ++y)
+                for (int x = 0; x <= c->kp.nx; ++x, ++m) h[size_t(c->pidx(x, y, 0))] = src[m];
+            HIP_TRY(hipMemcpy(dsts[k] + b->cases[size_t(i)].geo_base, h.data(), n * sizeof(double),
+                              hipMemcpyHostToDevice));
+        }
+        off += N;
+    }
+    return batch_start(b, nullptr);
 }
 
 int imexlb_batch_set_config(imexlb_batch *b, int block, int cells_per_thread, int persistent)

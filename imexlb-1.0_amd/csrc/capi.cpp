@@ -354,9 +354,17 @@ int imexlb_set_params(imexlb_ctx *c, double ome, double rho0, double uu)
 
 int imexlb_set_geometry(imexlb_ctx *c, double pos, double radius)
 {
+    return imexlb_set_geometry_shape(c, pos, radius, 1.0, 0.0);
+}
+
+int imexlb_set_geometry_shape(imexlb_ctx *c, double pos, double radius, double aspect, double angle_deg)
+{
     CHECK_CTX(c);
-    if (!std::isfinite(pos) || !std::isfinite(radius)) return fail(IMEXLB_E_ARG, "non-finite geometry");
-    c->g.build(pos, radius);
+    if (!std::isfinite(pos) || !std::isfinite(radius) || !std::isfinite(aspect) || !std::isfinite(angle_deg))
+        return fail(IMEXLB_E_ARG, "non-finite geometry");
+    if (aspect <= 0.0) return fail(IMEXLB_E_ARG, "ellipse axis ratio must be positive");
+    if (c->dim == 3 && (aspect != 1.0 || angle_deg != 0.0)) return fail(IMEXLB_E_ARG, "ellipses are 2D");
+    c->g.build(pos, radius, aspect, angle_deg);
     c->bc = resolve_boundary(c->g);
     c->geom_set = true;
     if (c->bc.undefined) return fail(IMEXLB_E_UB, c->bc.error);

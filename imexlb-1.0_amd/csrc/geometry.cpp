@@ -2,6 +2,7 @@
 #include "geometry.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <unordered_map>
 
@@ -36,13 +37,17 @@ void RefGeometry::init(int d, int nx, int ny, int nz)
 }
 
 // SetGeometry, initialization.f90:167-255 / initialization_3d.f90:170-279.
-void RefGeometry::build(double pos, double radius)
+void RefGeometry::build(double pos, double radius, double aspect, double angle_deg)
 {
     geo.assign(size_t(A), 0);
     fluid_id.clear(); bl.clear(); br.clear(); bu.clear(); bd.clear(); bf.clear(); bb.clear(); b_user.clear();
     scan_oob = 0;
     const int nx = n[0], ny = n[1], nz = n[2];
     const double yc = double(ny) / 2.0, zc = double(nz) / 2.0, r2 = radius * radius;
+    const bool ellipse = dim == 2 && (aspect != 1.0 || angle_deg != 0.0);
+    const double th = angle_deg * (3.14159265358979323846 / 180.0);
+    const double ct = std::cos(th), st = std::sin(th);
+    const double semi_a2 = radius * radius * aspect, semi_b2 = radius * radius / aspect;
     const int zlo = dim == 3 ? -1 : 0, zhi = dim == 3 ? L[2] + 1 : 0;
 
     // pass 1: interior mask + fluid list, ghost-ring wall lists (scan order z, y, x)
@@ -52,12 +57,17 @@ void RefGeometry::build(double pos, double radius)
                 const int64_t c = id(x, y, z);
                 if (interior(x, y, z)) {
                     const double a = double(x) - pos, b = double(y) - yc;
-                    double s = a * a + b * b;
-                    if (dim == 3) {
-                        const double w = double(z) - zc;
-                        s = s + w * w;
+                    if (ellipse) {
+                        const double u = ct * a + st * b, v = ct * b - st * a;
+                        geo[c] = u * u / semi_a2 + v * v / semi_b2 > 1.0 ? 1 : 0;
+                    } else {
+                        double s = a * a + b * b;
+                        if (dim == 3) {
+                            const double w = double(z) - zc;
+                            s = s + w * w;
+                        }
+                        geo[c] = s > r2 ? 1 : 0;
                     }
-                    geo[c] = s > r2 ? 1 : 0;
                     if (geo[c]) fluid_id.push_back(int32_t(c));
                     continue;
                 }

@@ -30,7 +30,12 @@ struct RefGeometry {
     int64_t scan_oob = 0;  // body-scan reads past the array that mattered
 
     void init(int d, int nx, int ny, int nz);
-    void build(double pos, double radius);
+    // body: the reference's circle/sphere of `radius` at (pos, ny/2[, nz/2]);
+    // in 2D optionally an ellipse of the same area, axis ratio `aspect`
+    // (semi-axes radius*sqrt(aspect) along, radius/sqrt(aspect) across its
+    // axis), the axis turned by `angle_deg` from x (aspect 1, angle 0 is the
+    // reference's test itself)
+    void build(double pos, double radius, double aspect = 1.0, double angle_deg = 0.0);
 
     int64_t id(int x, int y, int z) const {
         return int64_t(x + 3) + dy * int64_t(y + 3) + (dim == 3 ? dz * int64_t(z + 3) : 0);

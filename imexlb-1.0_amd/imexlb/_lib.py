@@ -32,6 +32,7 @@ SIGNATURES = {
     "imexlb_destroy": (C.c_int, [_P]),
     "imexlb_set_params": (C.c_int, [_P, C.c_double, C.c_double, C.c_double]),
     "imexlb_set_geometry": (C.c_int, [_P, C.c_double, C.c_double]),
+    "imexlb_set_geometry_shape": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_double]),
     "imexlb_geometry_size": (C.c_int, [_P, C.c_int, _I64P]),
     "imexlb_geometry_list": (C.c_int, [_P, C.c_int, _I32P, C.c_int64]),
     "imexlb_geometry_geo": (C.c_int, [_P, _I32P, C.c_int64]),
@@ -54,8 +55,10 @@ SIGNATURES = {
     "imexlb_profile_read": (C.c_int, [_P, _I64P, _DP]),
     "imexlb_info": (C.c_int, [_P, _I64P]),
     "imexlb_batch_create": (C.c_int, [C.POINTER(_P), C.c_int, _I32P, _DP, C.c_int, C.c_int]),
+    "imexlb_batch_create_shapes": (C.c_int, [C.POINTER(_P), C.c_int, _I32P, _DP, _DP, C.c_int, C.c_int]),
     "imexlb_batch_destroy": (C.c_int, [_P]),
     "imexlb_batch_init": (C.c_int, [_P, _DP]),
+    "imexlb_batch_init_fields": (C.c_int, [_P, _DP, _DP]),
     "imexlb_batch_advance": (C.c_int, [_P, C.c_int64]),
     "imexlb_batch_set_config": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
     "imexlb_batch_tune": (C.c_int, [_P, C.c_int, _I32P, _DP]),

@@ -27,7 +27,8 @@ _DP = C.POINTER(C.c_double)
 class Batch:
     """Handle on an imexlb_batch: cases given as Decks (2D)."""
 
-    def __init__(self, decks, device=0, precision=_lib.F64):
+    def __init__(self, decks, device=0, precision=_lib.F64, shapes=None):
+        """shapes: None (circles) or one (aspect, angle_deg) per case."""
         self.decks = list(decks)
         self.ncase = len(self.decks)
         nxny = np.zeros(2 * self.ncase, dtype=np.int32)
@@ -39,8 +40,15 @@ class Batch:
             nxny[2 * i:2 * i + 2] = dk.n[:2]
             params[5 * i:5 * i + 5] = (d.pos, d.radius, dk.ome, dk.rho0, d.uu)
         h = C.c_void_p()
-        check(lib().imexlb_batch_create(C.byref(h), self.ncase, nxny.ctypes.data_as(C.POINTER(C.c_int32)),
-                                        params.ctypes.data_as(_DP), device, precision), "imexlb_batch_create")
+        if shapes is None:
+            check(lib().imexlb_batch_create(C.byref(h), self.ncase, nxny.ctypes.data_as(C.POINTER(C.c_int32)),
+                                            params.ctypes.data_as(_DP), device, precision), "imexlb_batch_create")
+        else:
+            sh = np.ascontiguousarray(np.asarray(shapes, dtype=np.float64).reshape(self.ncase, 2))
+            check(lib().imexlb_batch_create_shapes(C.byref(h), self.ncase,
+                                                   nxny.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                   params.ctypes.data_as(_DP), sh.ctypes.data_as(_DP), device,
+                                                   precision), "imexlb_batch_create_shapes")
         self._h = h
         self.nodes = [(dk.n[0] + 1) * (dk.n[1] + 1) for dk in self.decks]
 
@@ -58,6 +66,16 @@ class Batch:
     def init(self, r_vec):
         r = np.ascontiguousarray(r_vec, dtype=np.float64)
         check(lib().imexlb_batch_init(self._h, r.ctypes.data_as(_DP)), "imexlb_batch_init")
+
+    def init_fields(self, ps, us):
+        """Warm start every case from its own p, u (interior layout, as
+        download_fields returns them)."""
+        p = np.ascontiguousarray(np.concatenate([np.ravel(x) for x in ps]), dtype=np.float64)
+        u = np.ascontiguousarray(np.concatenate([np.ravel(x) for x in us]), dtype=np.float64)
+        if p.size != sum(self.nodes) or u.size != 2 * sum(self.nodes):
+            raise ValueError("fields do not match the cases")
+        check(lib().imexlb_batch_init_fields(self._h, p.ctypes.data_as(_DP), u.ctypes.data_as(_DP)),
+              "imexlb_batch_init_fields")
 
     def advance(self, n):
         check(lib().imexlb_batch_advance(self._h, int(n)), "imexlb_batch_advance")

@@ -60,8 +60,12 @@ class Lattice:
     def set_params(self, ome, rho0, uu):
         check(lib().imexlb_set_params(self._h, ome, rho0, uu), "imexlb_set_params")
 
-    def set_geometry(self, pos, radius):
-        check(lib().imexlb_set_geometry(self._h, pos, radius), "imexlb_set_geometry")
+    def set_geometry(self, pos, radius, aspect=1.0, angle_deg=0.0):
+        if aspect == 1.0 and angle_deg == 0.0:
+            check(lib().imexlb_set_geometry(self._h, pos, radius), "imexlb_set_geometry")
+        else:
+            check(lib().imexlb_set_geometry_shape(self._h, pos, radius, aspect, angle_deg),
+                  "imexlb_set_geometry_shape")
 
     def geometry_list(self, name):
         n = C.c_int64()

@@ -96,6 +96,13 @@ int imexlb_set_params(imexlb_ctx *ctx, double ome, double rho0, double uu);
  * cylinder/sphere mask centred at (pos, ny/2[, nz/2]), boundary lists,
  * PassInt(geo), the body list b_user, and the device boundary tables. */
 int imexlb_set_geometry(imexlb_ctx *ctx, double pos, double radius);
+/* SetGeometry with an elliptic body (2D; no counterpart in the reference,
+ * for the warm-start study): the ellipse of the same area as the circle of
+ * `radius`, axis ratio `aspect` (semi-axes radius*sqrt(aspect) along its
+ * axis and radius/sqrt(aspect) across), axis turned by angle_deg from x.
+ * aspect 1, angle 0 is imexlb_set_geometry.  Everything after the mask
+ * (lists, body scan, tables) follows the reference's rules unchanged. */
+int imexlb_set_geometry_shape(imexlb_ctx *ctx, double pos, double radius, double aspect, double angle_deg);
 /* Sizes / contents of the lists SetGeometry builds, as 0-based reference
  * cell ids (int32, like the reference's integer arrays). */
 int imexlb_geometry_size(imexlb_ctx *ctx, int list, int64_t *n);
@@ -185,9 +192,18 @@ typedef struct imexlb_batch imexlb_batch;
  * IMEXLB_F64, IMEXLB_F32 or IMEXLB_F16_F32. */
 int imexlb_batch_create(imexlb_batch **b, int ncase, const int32_t *nxny, const double *params, int device,
                         int precision);
+/* The same with an elliptic body per case: shapes[2*i..2*i+1] = (aspect,
+ * angle_deg) as imexlb_set_geometry_shape; NULL: all circles. */
+int imexlb_batch_create_shapes(imexlb_batch **b, int ncase, const int32_t *nxny, const double *params,
+                               const double *shapes, int device, int precision);
 int imexlb_batch_destroy(imexlb_batch *b);
 /* InitUP + InitPDF + the first Collision of every case (same r_vec). */
 int imexlb_batch_init(imexlb_batch *b, const double *r_vec);
+/* Warm start: InitPDF + the first Collision of every case from given
+ * fields, p and u of all cases concatenated in case order, each in the
+ * interior layout of imexlb_batch_download_fields ((nx+1)(ny+1) values of
+ * p; u[d][y][x]). */
+int imexlb_batch_init_fields(imexlb_batch *b, const double *p, const double *u);
 /* nsteps complete iterations of every case. */
 int imexlb_batch_advance(imexlb_batch *b, int64_t nsteps);
 /* Step-kernel shape: threads per block (64, 128, 256, 512), nodes per

@@ -247,9 +247,22 @@ void orc_pass_f(orc *o, double *a)
  * independent analysis restatement, the circle/sphere test on every node,
  * and structural invariants of the lists (tests/test_oracle_setup.py).
  * ---------------------------------------------------------------------- */
-void orc_set_geometry(orc *o, double pos, double radius)
+void orc_set_geometry_shape(orc *o, double pos, double radius, double aspect, double angle_deg);
+void orc_set_geometry(orc *o, double pos, double radius) { orc_set_geometry_shape(o, pos, radius, 1.0, 0.0); }
+
+/* The same with an elliptic body (2D, no counterpart in the reference): the
+ * ellipse of the circle's area with axis ratio `aspect`, axis turned by
+ * angle_deg from x; a node is fluid when (u/A)^2 + (v/B)^2 > 1 in the turned
+ * frame, A = radius*sqrt(aspect), B = radius/sqrt(aspect).  aspect 1,
+ * angle 0 is the reference's circle test itself.  Pinned by closed forms:
+ * area, symmetry, aspect 1 == circle, (aspect, 90 deg) == (1/aspect, 0). */
+void orc_set_geometry_shape(orc *o, double pos, double radius, double aspect, double angle_deg)
 {
     const int dim = o->dim;
+    const int ellipse = dim == 2 && (aspect != 1.0 || angle_deg != 0.0);
+    const double th = angle_deg * (3.14159265358979323846 / 180.0);
+    const double ct = cos(th), st = sin(th);
+    const double semi_a2 = radius * radius * aspect, semi_b2 = radius * radius / aspect;
     const int nx = o->n[0], ny = o->n[1], nz = o->n[2];
     o->pos = pos;
     o->radius = radius;
@@ -271,7 +284,12 @@ void orc_set_geometry(orc *o, double pos, double radius)
                         double c = (double)z - (double)nz / 2.0;
                         s = s + c * c;
                     }
-                    if (s > radius * radius) {
+                    int fluid = s > radius * radius;
+                    if (ellipse) {
+                        double u = ct * a + st * b, v = ct * b - st * a;
+                        fluid = u * u / semi_a2 + v * v / semi_b2 > 1.0;
+                    }
+                    if (fluid) {
                         o->geo[id] = 1;
                         o->fluid_id[idn++] = id;
                     }

@@ -36,6 +36,7 @@ def lib():
         L.orc_destroy.argtypes = [P]
         L.orc_set_params.argtypes = [P, C.c_double, C.c_double, C.c_double]
         L.orc_set_geometry.argtypes = [P, C.c_double, C.c_double]
+        L.orc_set_geometry_shape.argtypes = [P, C.c_double, C.c_double, C.c_double, C.c_double]
         L.orc_init_up.argtypes = [P, C.POINTER(C.c_double)]
         for name in ("orc_init_pdf", "orc_collision", "orc_boundary_condition",
                      "orc_propagation", "orc_post_processing", "orc_moments", "orc_post_bc"):
@@ -177,8 +178,11 @@ class Oracle:
     def set_params(self, ome, rho0, uu):
         lib().orc_set_params(self._h, ome, rho0, uu)
 
-    def set_geometry(self, pos, radius):
-        lib().orc_set_geometry(self._h, pos, radius)
+    def set_geometry(self, pos, radius, aspect=1.0, angle_deg=0.0):
+        if aspect == 1.0 and angle_deg == 0.0:
+            lib().orc_set_geometry(self._h, pos, radius)
+        else:
+            lib().orc_set_geometry_shape(self._h, pos, radius, aspect, angle_deg)
 
     def init_up(self, r_vec):
         r = np.ascontiguousarray(r_vec, dtype=np.float64)

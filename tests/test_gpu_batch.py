@@ -91,3 +91,55 @@ def test_batch_fp16_close_to_fp64(r_vec):
         assert err < 1e-2, (i, err)
     b64.close()
     b16.close()
+
+
+@pytest.mark.gpu
+def test_batch_ellipses_match_single_lattices(r_vec):
+    """Cases with elliptic bodies (imexlb_batch_create_shapes) evolve like
+    the same shapes run as single lattices, bit for bit."""
+    decks = _small_cases()[:3]
+    shapes = [(1.8, 20.0), (0.6, 70.0), (1.0, 0.0)]
+    b = Batch(decks, precision=I.F64, shapes=shapes)
+    b.init(r_vec)
+    b.advance(25)
+    for i, (dk, (aspect, angle)) in enumerate(zip(decks, shapes)):
+        d = I.derive(dk)
+        lat = I.Lattice(2, dk.n[0], dk.n[1], 0, device=0)
+        lat.set_params(dk.ome, dk.rho0, d.uu)
+        lat.set_geometry(d.pos, d.radius, aspect, angle)
+        lat.init_up(r_vec)
+        lat.init_pdf()
+        lat.advance(25)
+        p, u = lat.download_fields()
+        pb, ub = b.download_fields(i)
+        assert np.isfinite(p).all() and np.array_equal(p, pb) and np.array_equal(u, ub), i
+        lat.close()
+    b.close()
+
+
+@pytest.mark.gpu
+def test_batch_warm_start_matches_single_lattices(r_vec):
+    """imexlb_batch_init_fields (warm start from given p, u) equals the
+    single-lattice warm-start hook (imexlb_upload_fields + InitPDF)."""
+    decks = _small_cases()[:3]
+    rng = np.random.default_rng(5)
+    ps, us = [], []
+    for dk in decks:
+        n = (dk.n[0] + 1) * (dk.n[1] + 1)
+        ps.append(rng.normal(0.0, 1e-3, n))
+        us.append(np.concatenate([0.05 + rng.normal(0.0, 5e-3, n), rng.normal(0.0, 5e-3, n)]))
+    b = Batch(decks, precision=I.F64)
+    b.init_fields(ps, us)
+    b.advance(21)
+    Fb = b.forces()
+    for i, dk in enumerate(decks):
+        lat, _ = I.setup(dk, r_vec, device=0)
+        lat.upload_fields(ps[i], us[i])
+        lat.init_pdf()
+        lat.advance(21)
+        p, u = lat.download_fields()
+        pb, ub = b.download_fields(i)
+        assert np.isfinite(p).all() and np.array_equal(p, pb) and np.array_equal(u, ub), i
+        assert np.array_equal(lat.forces(), Fb[i]), i
+        lat.close()
+    b.close()

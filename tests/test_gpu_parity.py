@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import imexlb as I
+import oracle as O
 from cases import assert_state_equal, get_deck, oracle_setup
 
 pytestmark = pytest.mark.gpu
@@ -118,6 +119,30 @@ def test_separate_fill_path_matches_oracle(r_vec, monkeypatch):
         assert_state_equal(lat, o)
         lat.close()
         o.close()
+
+
+@pytest.mark.parametrize("aspect,angle", [(1.8, 35.0), (0.5, 80.0)])
+def test_ellipse_bitwise(aspect, angle, r_vec):
+    """An elliptic body (the warm-start study's second shape): 60 steps equal
+    the oracle bit for bit."""
+    nx, ny, pos, radius, ome, uu = 64, 32, 20.0, 5.0, 1.7, 0.05
+    print(f"[parity] ellipse aspect={aspect} angle={angle}", flush=True)
+    lat = I.Lattice(2, nx, ny, 0, device=0)
+    lat.set_params(ome, 1.0, uu)
+    lat.set_geometry(pos, radius, aspect, angle)
+    lat.init_up(r_vec)
+    lat.init_pdf()
+    o = O.Oracle(2, nx, ny, 0)
+    o.set_params(ome, 1.0, uu)
+    o.set_geometry(pos, radius, aspect, angle)
+    o.init_up(r_vec)
+    o.init_pdf()
+    lat.advance(60)
+    o.run(0, 59, 0)
+    assert lat.monitor() == o.monitor()
+    assert_state_equal(lat, o)
+    lat.close()
+    o.close()
 
 
 def test_graph_replay_matches_oracle(r_vec):

@@ -215,3 +215,28 @@ def test_state_comparison_catches_one_differing_value(r_vec):
             assert_state_equal(s, o, chunk=97)
         arr[idx] = keep
     o.close()
+
+
+@pytest.mark.parametrize("aspect,angle", [(2.0, 0.0), (0.6, 40.0), (1.5, 90.0)])
+def test_ellipse_geometry_matches_oracle(aspect, angle):
+    """The library's elliptic mask, lists and body scan equal the oracle's."""
+    nx, ny, pos, radius = 80, 40, 24.0, 7.0
+    lat = I.Lattice(2, nx, ny, 0, device=-1)
+    lat.set_params(1.5, 1.0, 0.05)
+    lat.set_geometry(pos, radius, aspect, angle)
+    o = O.Oracle(2, nx, ny, 0)
+    o.set_params(1.5, 1.0, 0.05)
+    o.set_geometry(pos, radius, aspect, angle)
+    assert np.array_equal(lat.geometry_geo(), o.geo)
+    for name in ("fluid_id", "b_user", "bl", "br", "bu", "bd"):
+        assert np.array_equal(lat.geometry_list(name), o.list(name)), name
+    lat.close()
+    o.close()
+
+
+def test_ellipse_rejected_in_3d():
+    lat = I.Lattice(3, 8, 8, 8, device=-1)
+    lat.set_params(1.5, 1.0, 0.05)
+    with pytest.raises(I.ImexlbError):
+        lat.set_geometry(4.0, 2.0, 2.0, 0.0)
+    lat.close()

@@ -204,3 +204,49 @@ def test_init_up_closed_form(r_vec):
         assert o.u[2 * c + 1] == 0.0 and o.p[c] == 0.0
     assert o.u[2 * o.id(80, 50)] == 0.0  # cylinder centre is solid
     o.close()
+
+
+# ---------------------------------------------------------------------------
+# elliptic bodies (an extension for the warm-start study; no reference)
+
+def _ellipse_oracle(aspect, angle, nx=96, ny=48, pos=40.0, radius=9.0):
+    o = O.Oracle(2, nx, ny, 0)
+    o.set_params(1.5, 1.0, 0.05)
+    o.set_geometry(pos, radius, aspect, angle)
+    return o
+
+
+def _solid_mask(o):
+    nx, ny = o.n[0], o.n[1]
+    ids = np.array([[o.id(x, y, 0) for x in range(nx + 1)] for y in range(ny + 1)])
+    return o.geo[ids] == 0
+
+
+def test_ellipse_unit_aspect_is_the_circle():
+    a = _ellipse_oracle(1.0, 0.0)
+    b = O.Oracle(2, 96, 48, 0)
+    b.set_params(1.5, 1.0, 0.05)
+    b.set_geometry(40.0, 9.0)
+    assert np.array_equal(_solid_mask(a), _solid_mask(b))
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("aspect,angle", [(2.0, 0.0), (0.5, 30.0), (1.7, 75.0)])
+def test_ellipse_area_and_symmetry(aspect, angle):
+    """Solid node count ~ pi r^2 (the ellipse keeps the circle's area), and
+    the mask is symmetric under the half turn about the centre (40, 24)."""
+    o = _ellipse_oracle(aspect, angle)
+    m = _solid_mask(o)
+    assert abs(m.sum() - np.pi * 81.0) < 0.06 * np.pi * 81.0, m.sum()
+    # node (x, y) <-> (80 - x, 48 - y)
+    assert np.array_equal(m[:, :81], m[::-1, 80::-1])
+    o.close()
+
+
+def test_ellipse_quarter_turn_swaps_the_axes():
+    """(aspect, 90 deg) and (1/aspect, 0 deg) describe the same ellipse; cos 90
+    deg is not exactly 0 in floating point, so allow a node or two to differ."""
+    a = _solid_mask(_ellipse_oracle(2.0, 90.0))
+    b = _solid_mask(_ellipse_oracle(0.5, 0.0))
+    assert (a != b).sum() <= 2

@@ -66,6 +66,21 @@ SIGNATURES = {
     "imexlb_batch_forces": (C.c_int, [_P, _DP]),
     "imexlb_batch_download_fields": (C.c_int, [_P, C.c_int, _DP, _DP]),
     "imexlb_batch_info": (C.c_int, [_P, _I64P]),
+    # warm-start network kernels (include/imexlb_nn.h); float pointers are device addresses
+    "imexlb_nn_conv3x3": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
+    "imexlb_nn_flip_weights": (C.c_int, [_P, _P, C.c_int, C.c_int, _P]),
+    "imexlb_nn_conv3x3_dw": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
+    "imexlb_nn_bias_grad": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
+    "imexlb_nn_relu_bwd": (C.c_int, [_P, _P, _P, C.c_int64, _P]),
+    "imexlb_nn_pool2": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
+    "imexlb_nn_pool2_bwd": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
+    "imexlb_nn_up2": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
+    "imexlb_nn_up2_bwd": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
+    "imexlb_nn_copy_channels": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                          C.c_int, C.c_int, _P]),
+    "imexlb_nn_mse": (C.c_int, [_P, _P, _P, C.c_int64, _P, _P]),
+    "imexlb_nn_adam": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
+                                 _P]),
 }
 
 

@@ -1,0 +1,476 @@
+"""Learned warm start for D2Q9 cylinder/ellipse cases.
+
+    python -m imexlb.surrogate [--train 2000] [--test 200] [--epochs 15]
+
+1. Cases: 512x128 channel, a body of the cylinder's area (radius Ny/10) at
+   x = 0.2 Nx, a circle or an ellipse of axis ratio U[0.5, 2] turned by
+   U[0, 90] degrees; Re log-uniform in [40, 300]; inlet velocity 0.05.
+2. Data: every case run by the batched solver (FP16 storage) to a developed
+   wake; the target is its (p, u, v) field averaged over the last snapshots.
+3. Network: a 4-level convolutional encoder-decoder (3x3 convolutions with
+   ReLU, 2x2 average pooling, nearest upsampling, skip connections), FP32,
+   trained with Adam by the HIP kernels of include/imexlb_nn.h.  Input:
+   body mask, normalised log Re, and the x and y coordinates.
+4. Evaluation on held-out cases: the solver started cold (InitUP) and
+   warm (the predicted field through imexlb_batch_init_fields), and warm from
+   the training set's mean field (the method without the network); for each
+   the number of steps until the drag's running mean stays within 1 % of the
+   converged cold-start value, and whether the converged drag and lift match
+   the cold run's.
+
+Torch supplies device memory only; every arithmetic operation of the
+network is one of the library's kernels.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import lib
+from .inputs import Deck, derive
+
+NX, NY = 512, 128           # lattice (nodes 0..NX x 0..NY); the network sees NY x NX
+UU = 0.05
+RE_RANGE = (40.0, 300.0)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _ok(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed ({rc})")
+
+
+def layer_shapes(cin, cout, chans):
+    """(CO, CI) of every convolution: encoder pairs, decoder pairs, output."""
+    shapes, prev = [], cin
+    for c in chans:
+        shapes += [(c, prev), (c, c)]
+        prev = c
+    for l in range(len(chans) - 2, -1, -1):
+        shapes += [(chans[l], chans[l + 1] + chans[l]), (chans[l], chans[l])]
+    shapes.append((cout, chans[0]))
+    return shapes
+
+
+class UNet:
+    """Encoder-decoder on device buffers; batch N of H x W fields."""
+
+    def __init__(self, cin, cout, chans, N, H, W, seed=0, params=None):
+        import torch
+        self.t = torch
+        self.cin, self.cout, self.chans, self.N, self.H, self.W = cin, cout, list(chans), N, H, W
+        L = len(self.chans)
+        if H % (1 << (L - 1)) or W % (1 << (L - 1)):
+            raise ValueError("H and W must be divisible by 2^(levels-1)")
+        self.shapes = layer_shapes(cin, cout, self.chans)
+        rng = np.random.default_rng(seed)
+        dev = "cuda"
+        f32 = torch.float32
+        self.w, self.b = [], []
+        for k, (co, ci) in enumerate(self.shapes):
+            if params is not None:
+                w0, b0 = params[k]
+            else:
+                w0 = rng.normal(0.0, np.sqrt(2.0 / (9 * ci)), (co, ci, 3, 3))
+                b0 = np.zeros(co)
+            self.w.append(torch.tensor(np.asarray(w0, np.float32), device=dev))
+            self.b.append(torch.tensor(np.asarray(b0, np.float32), device=dev))
+        z = lambda like: torch.zeros_like(like)
+        self.dw = [z(w) for w in self.w]
+        self.db = [z(b) for b in self.b]
+        self.mw = [z(w) for w in self.w]
+        self.vw = [z(w) for w in self.w]
+        self.mb = [z(b) for b in self.b]
+        self.vb = [z(b) for b in self.b]
+        self.wt = [torch.empty((ci, co, 3, 3), dtype=f32, device=dev) for co, ci in self.shapes]
+        self.step_count = 0
+        # per-level sizes and buffers
+        self.hw = [(H >> l, W >> l) for l in range(L)]
+        e = lambda c, l: torch.empty((N, c, *self.hw[l]), dtype=f32, device=dev)
+        self.level_of = []  # spatial level of every conv
+        for l in range(L):
+            self.level_of += [l, l]
+        for l in range(L - 2, -1, -1):
+            self.level_of += [l, l]
+        self.level_of.append(0)
+        self.out = [e(co, self.level_of[k]) for k, (co, ci) in enumerate(self.shapes)]
+        self.dz = [e(co, self.level_of[k]) for k, (co, ci) in enumerate(self.shapes)]
+        self.dx = [e(ci, self.level_of[k]) for k, (co, ci) in enumerate(self.shapes)]
+        self.pooled = [None] + [e(self.chans[l - 1], l) for l in range(1, L)]
+        self.cat = [e(self.chans[l + 1] + self.chans[l], l) for l in range(L - 1)]
+        self.denc = [e(self.chans[l], l) for l in range(L - 1)]
+        self.dpool = [None] + [e(self.chans[l - 1], l - 1) for l in range(1, L)]
+        self.dup = [e(self.chans[l + 1], l + 1) for l in range(L - 1)]
+        self.inp = [None] * len(self.shapes)
+        self.loss_buf = torch.zeros(1, dtype=f32, device=dev)
+        self.grad_out = e(cout, 0)
+
+    # ---------------------------------------------------------------- layers
+    def _conv(self, k, x, relu=True):
+        co, ci = self.shapes[k]
+        h, w = self.hw[self.level_of[k]]
+        self.inp[k] = x
+        _ok(lib().imexlb_nn_conv3x3(_ptr(x), _ptr(self.w[k]), _ptr(self.b[k]), _ptr(self.out[k]), self.N, ci, co, h,
+                                    w, 1 if relu else 0, None), "conv3x3")
+        return self.out[k]
+
+    def _conv_back(self, k, g, relu=True, need_dx=True):
+        co, ci = self.shapes[k]
+        h, w = self.hw[self.level_of[k]]
+        n = self.N * co * h * w
+        if relu:
+            _ok(lib().imexlb_nn_relu_bwd(_ptr(self.out[k]), _ptr(g), _ptr(self.dz[k]), n, None), "relu_bwd")
+            dz = self.dz[k]
+        else:
+            dz = g
+        _ok(lib().imexlb_nn_conv3x3_dw(_ptr(self.inp[k]), _ptr(dz), _ptr(self.dw[k]), self.N, ci, co, h, w, None),
+            "conv3x3_dw")
+        _ok(lib().imexlb_nn_bias_grad(_ptr(dz), _ptr(self.db[k]), self.N, co, h, w, None), "bias_grad")
+        if not need_dx:
+            return None
+        _ok(lib().imexlb_nn_flip_weights(_ptr(self.w[k]), _ptr(self.wt[k]), ci, co, None), "flip")
+        _ok(lib().imexlb_nn_conv3x3(_ptr(dz), _ptr(self.wt[k]), None, _ptr(self.dx[k]), self.N, co, ci, h, w, 0,
+                                    None), "conv3x3 (data gradient)")
+        return self.dx[k]
+
+    # ------------------------------------------------------------------ passes
+    def forward(self, x):
+        """x: (N, cin, H, W) float32 device tensor; returns the output buffer."""
+        L, ch, N = len(self.chans), self.chans, self.N
+        k, h, enc = 0, x, []
+        for l in range(L):
+            if l > 0:
+                hh, ww = self.hw[l]
+                _ok(lib().imexlb_nn_pool2(_ptr(enc[l - 1]), _ptr(self.pooled[l]), N, ch[l - 1], hh, ww, None), "pool2")
+                h = self.pooled[l]
+            h = self._conv(k, h)
+            h = self._conv(k + 1, h)
+            k += 2
+            enc.append(h)
+        d = enc[-1]
+        for l in range(L - 2, -1, -1):
+            hh, ww = self.hw[l + 1]
+            ctot = ch[l + 1] + ch[l]
+            _ok(lib().imexlb_nn_up2(_ptr(d), _ptr(self.cat[l]), N, ch[l + 1], hh, ww, ctot, 0, None), "up2")
+            H0, W0 = self.hw[l]
+            _ok(lib().imexlb_nn_copy_channels(_ptr(enc[l]), ch[l], 0, _ptr(self.cat[l]), ctot, ch[l + 1], N, ch[l], H0,
+                                              W0, 0, None), "concat")
+            d = self._conv(k, self.cat[l])
+            d = self._conv(k + 1, d)
+            k += 2
+        return self._conv(k, d, relu=False)
+
+    def backward(self, g):
+        """g: gradient of the loss w.r.t. the output; fills dw, db."""
+        L, ch, N = len(self.chans), self.chans, self.N
+        for t in self.dw:
+            t.zero_()
+        k = len(self.shapes) - 1
+        g = self._conv_back(k, g, relu=False)
+        k -= 1
+        for l in range(0, L - 1):
+            g = self._conv_back(k, g)
+            g = self._conv_back(k - 1, g)
+            k -= 2
+            ctot, cu = ch[l + 1] + ch[l], ch[l + 1]
+            H0, W0 = self.hw[l]
+            _ok(lib().imexlb_nn_copy_channels(_ptr(g), ctot, cu, _ptr(self.denc[l]), ch[l], 0, N, ch[l], H0, W0, 0,
+                                              None), "split")
+            hh, ww = self.hw[l + 1]
+            _ok(lib().imexlb_nn_up2_bwd(_ptr(g), _ptr(self.dup[l]), N, cu, hh, ww, ctot, 0, None), "up2_bwd")
+            g = self.dup[l]
+        for l in range(L - 1, -1, -1):
+            g = self._conv_back(k, g)
+            g = self._conv_back(k - 1, g, need_dx=(k - 1) > 0)
+            k -= 2
+            if l > 0:
+                hh, ww = self.hw[l]
+                _ok(lib().imexlb_nn_pool2_bwd(_ptr(g), _ptr(self.dpool[l]), N, ch[l - 1], hh, ww, None), "pool2_bwd")
+                H0, W0 = self.hw[l - 1]
+                _ok(lib().imexlb_nn_copy_channels(_ptr(self.denc[l - 1]), ch[l - 1], 0, _ptr(self.dpool[l]), ch[l - 1],
+                                                  0, N, ch[l - 1], H0, W0, 1, None), "skip gradient")
+                g = self.dpool[l]
+
+    def loss(self, pred, target, with_grad=True):
+        self.loss_buf.zero_()
+        _ok(lib().imexlb_nn_mse(_ptr(pred), _ptr(target), _ptr(self.grad_out) if with_grad else None, pred.numel(),
+                                _ptr(self.loss_buf), None), "mse")
+        return self.loss_buf
+
+    def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8):
+        self.step_count += 1
+        for group in ((self.w, self.dw, self.mw, self.vw), (self.b, self.db, self.mb, self.vb)):
+            for p, g, m, v in zip(*group):
+                _ok(lib().imexlb_nn_adam(_ptr(p), _ptr(g), _ptr(m), _ptr(v), p.numel(), lr, b1, b2, eps,
+                                         self.step_count, None), "adam")
+
+    def params_numpy(self):
+        return [(w.cpu().numpy().astype(np.float64), b.cpu().numpy().astype(np.float64)) for w, b in zip(self.w, self.b)]
+
+    def n_params(self):
+        return int(sum(w.numel() + b.numel() for w, b in zip(self.w, self.b)))
+
+
+# --------------------------------------------------------------------- cases
+def make_cases(n, seed):
+    """Seeded cases: list of (Deck, aspect, angle_deg)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    radius = NY / 10.0
+    D = 2.0 * radius
+    for _ in range(n):
+        re = float(np.exp(rng.uniform(np.log(RE_RANGE[0]), np.log(RE_RANGE[1]))))
+        nu = UU * D / re
+        ome = 1.0 / (3.0 * nu + 0.5)
+        ellipse = rng.random() < 0.5
+        aspect = float(rng.uniform(0.5, 2.0)) if ellipse else 1.0
+        angle = float(rng.uniform(0.0, 90.0)) if ellipse else 0.0
+        out.append((Deck(2, (NX, NY, 0), D / NX, 0.2, re, ome, 1.0, 1.0, 1.0, 0.01), aspect, angle))
+    return out
+
+
+def _batch(cases, precision):
+    from .ensemble import Batch
+    return Batch([c[0] for c in cases], precision=precision, shapes=[(c[1], c[2]) for c in cases])
+
+
+def solid_masks(cases):
+    """Body masks (1 = solid) on the NY x NX network grid, from the library's geometry."""
+    from .lattice import Lattice
+    out = np.zeros((len(cases), NY, NX), np.float32)
+    for i, (dk, a, ang) in enumerate(cases):
+        d = derive(dk)
+        lat = Lattice(2, NX, NY, 0, device=-1)
+        lat.set_params(dk.ome, dk.rho0, d.uu)
+        lat.set_geometry(d.pos, d.radius, a, ang)
+        geo = lat.geometry_geo().reshape(NY + 7, NX + 7)[3:3 + NY, 3:3 + NX]
+        out[i] = (geo == 0).astype(np.float32)
+        lat.close()
+    return out
+
+
+def inputs(cases, masks):
+    """Network input (M, 4, NY, NX): mask, normalised log Re, x, y."""
+    M = len(cases)
+    x = np.zeros((M, 4, NY, NX), np.float32)
+    lre = np.log([c[0].Re for c in cases])
+    mid, half = np.log(RE_RANGE).mean(), np.ptp(np.log(RE_RANGE)) / 2
+    x[:, 0] = masks
+    x[:, 1] = ((lre - mid) / half)[:, None, None]
+    x[:, 2] = np.linspace(0.0, 1.0, NX, dtype=np.float32)[None, None, :]
+    x[:, 3] = np.linspace(0.0, 1.0, NY, dtype=np.float32)[None, :, None]
+    return x
+
+
+def developed_fields(cases, steps, avg_snapshots, avg_every, r_vec, precision, log=None):
+    """Run every case (batched) for `steps`; return (M, 3, NY, NX) float32 of
+    (p / uu^2, u / uu, v / uu) averaged over the last snapshots, and the
+    final forces."""
+    b = _batch(cases, precision)
+    b.init(r_vec)
+    first = steps - (avg_snapshots - 1) * avg_every
+    b.advance(first)
+    acc = np.zeros((len(cases), 3, NY, NX))
+    for s in range(avg_snapshots):
+        if s:
+            b.advance(avg_every)
+        for i in range(len(cases)):
+            p, u = b.download_fields(i)
+            n = (NX + 1) * (NY + 1)
+            acc[i, 0] += p.reshape(NY + 1, NX + 1)[:NY, :NX] / UU ** 2
+            acc[i, 1] += u[:n].reshape(NY + 1, NX + 1)[:NY, :NX] / UU
+            acc[i, 2] += u[n:].reshape(NY + 1, NX + 1)[:NY, :NX] / UU
+        if log:
+            log(f"[surrogate] snapshot {s + 1}/{avg_snapshots}")
+    F = b.forces()
+    b.close()
+    return (acc / avg_snapshots).astype(np.float32), F
+
+
+def to_lattice_fields(pred, r_vec):
+    """Network output (3, NY, NX) -> p, u of the (NY+1) x (NX+1) lattice
+    (interior layout), the last row and column repeated.  u_x carries the
+    same perturbation InitUP gives a cold start (factor 1 + 0.4 (r - 0.5),
+    r = r_vec(mod(id, 100)), initialization.f90:273), so both starts seed
+    vortex shedding alike."""
+    full = np.pad(pred.astype(np.float64), ((0, 0), (0, 1), (0, 1)), mode="edge")
+    y, x = np.mgrid[0:NY + 1, 0:NX + 1]
+    ref_id = (x + 3) + (NX + 7) * (y + 3)
+    r = 1.0 + ((np.asarray(r_vec)[ref_id % 100] - 0.5) * 2.0) * 0.2
+    p = full[0] * UU ** 2
+    u = np.concatenate([(full[1] * UU * r).ravel(), (full[2] * UU).ravel()])
+    return p.ravel(), u
+
+
+def convergence_step(t, drag, ref, window, tol=0.01):
+    """First logged time after which the running mean of drag (over `window`
+    steps) stays within tol of ref; None if never."""
+    w = max(1, int(round(window / (t[1] - t[0]))))
+    if len(drag) < w:
+        return None
+    rm = np.convolve(drag, np.ones(w) / w, mode="valid")
+    tt = t[w - 1:]
+    ok = np.abs(rm - ref) <= tol * abs(ref)
+    bad = np.nonzero(~ok)[0]
+    if bad.size == 0:
+        return int(tt[0])
+    if bad[-1] + 1 >= len(ok):
+        return None
+    return int(tt[bad[-1] + 1])
+
+
+def run_history(cases, steps, every, precision, r_vec=None, fields=None):
+    """Batched run from InitUP (r_vec) or from given fields; drag/lift every
+    `every` steps.  Returns (t, F[steps/every, M, 2], seconds)."""
+    b = _batch(cases, precision)
+    if fields is None:
+        b.init(r_vec)
+    else:
+        b.init_fields([f[0] for f in fields], [f[1] for f in fields])
+    b.synchronize()
+    t0 = time.perf_counter()
+    hist, ts = [], []
+    for s in range(every, steps + 1, every):
+        b.advance(every)
+        hist.append(b.forces())
+        ts.append(s)
+    sec = time.perf_counter() - t0
+    b.close()
+    return np.array(ts), np.array(hist), sec
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=2000)
+    ap.add_argument("--test", type=int, default=200)
+    ap.add_argument("--data-steps", type=int, default=16000)
+    ap.add_argument("--snapshots", type=int, default=8)
+    ap.add_argument("--snapshot-every", type=int, default=500)
+    ap.add_argument("--epochs", type=int, default=15)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--chans", default="16,32,64,128")
+    ap.add_argument("--eval-steps", type=int, default=40000)
+    ap.add_argument("--seed", type=int, default=2024)
+    a = ap.parse_args(argv)
+    import torch
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    prec = _lib.F16_F32
+    r_vec = np.random.default_rng(a.seed).random(100)
+    cases = make_cases(a.train + a.test, a.seed)
+    train, test = cases[:a.train], cases[a.train:]
+    masks = solid_masks(cases)
+    X = inputs(cases, masks)
+    log(f"[surrogate] data: {len(cases)} cases, {a.data_steps} steps")
+    t0 = time.perf_counter()
+    Y, _ = developed_fields(cases, a.data_steps, a.snapshots, a.snapshot_every, r_vec, prec, log)
+    t_data = time.perf_counter() - t0
+    finite = np.isfinite(Y).all(axis=(1, 2, 3))
+    tr_idx = np.nonzero(finite[:a.train])[0]
+    log(f"[surrogate] data in {t_data:.1f} s; {int((~finite).sum())} diverged cases left out")
+    Xd = torch.tensor(X, device="cuda")
+    Yd = torch.tensor(np.nan_to_num(Y), device="cuda")
+    net = UNet(4, 3, [int(c) for c in a.chans.split(",")], a.batch, NY, NX, seed=a.seed)
+    xb = torch.empty((a.batch, 4, NY, NX), device="cuda")
+    yb = torch.empty((a.batch, 3, NY, NX), device="cuda")
+    rng = np.random.default_rng(a.seed + 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    steps_per_epoch = len(tr_idx) // a.batch
+    curve = []
+    for ep in range(a.epochs):
+        perm = rng.permutation(tr_idx)
+        lr = a.lr * (0.5 ** (ep // 5))
+        tot = 0.0
+        for s in range(steps_per_epoch):
+            idx = torch.tensor(perm[s * a.batch:(s + 1) * a.batch], device="cuda")
+            xb.copy_(Xd.index_select(0, idx))
+            yb.copy_(Yd.index_select(0, idx))
+            out = net.forward(xb)
+            net.loss(out, yb)
+            net.backward(net.grad_out)
+            net.adam(lr)
+            if s % 25 == 0:
+                tot = float(net.loss_buf.item())
+        curve.append(tot)
+        log(f"[surrogate] epoch {ep + 1}/{a.epochs} loss {tot:.5f}")
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+
+    # predictions for the held-out cases
+    te_idx = np.arange(a.train, a.train + a.test)
+    preds = np.zeros((a.test, 3, NY, NX), np.float32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(0, a.test, a.batch):
+        sel = te_idx[s:s + a.batch]
+        sel = np.concatenate([sel, np.full(a.batch - len(sel), sel[-1])])
+        xb.copy_(Xd.index_select(0, torch.tensor(sel, device="cuda")))
+        out = net.forward(xb)
+        k = min(a.batch, a.test - s)
+        preds[s:s + k] = out[:k].cpu().numpy()
+    torch.cuda.synchronize()
+    t_infer = time.perf_counter() - t0
+    held = Y[te_idx]
+    ok = finite[te_idx]
+    err = float(np.sqrt(np.mean((preds[ok] - held[ok]) ** 2)) / np.sqrt(np.mean(held[ok] ** 2)))
+    mean_field = Y[tr_idx].mean(axis=0)
+    err_mean = float(np.sqrt(np.mean((mean_field[None] - held[ok]) ** 2)) / np.sqrt(np.mean(held[ok] ** 2)))
+    log(f"[surrogate] held-out rel. RMS error: network {err:.3f}, mean field {err_mean:.3f}")
+
+    # cold, warm (network), warm (mean field)
+    solid = masks[te_idx][:, None]
+    warm = [to_lattice_fields(np.where(solid[i] > 0, 0.0, preds[i]), r_vec) for i in range(a.test)]
+    meanw = [to_lattice_fields(np.where(solid[i] > 0, 0.0, mean_field), r_vec) for i in range(a.test)]
+    every = 100
+    tc, Fc, sc = run_history(test, a.eval_steps, every, prec, r_vec=r_vec)
+    tw, Fw, sw = run_history(test, a.eval_steps, every, prec, fields=warm)
+    tm, Fm, sm = run_history(test, a.eval_steps, every, prec, fields=meanw)
+    window = 2560  # about one shedding period at St 0.2
+    tail = max(1, len(tc) // 4)
+    res = []
+    for i in range(a.test):
+        if not ok[i] or not np.isfinite(Fc[:, i]).all():
+            continue
+        ref = Fc[-tail:, i, 0].mean()
+        lift_c = Fc[-tail:, i, 1].std()
+        r = {"re": test[i][0].Re, "cold": convergence_step(tc, Fc[:, i, 0], ref, window),
+             "warm": convergence_step(tw, Fw[:, i, 0], ref, window),
+             "mean": convergence_step(tm, Fm[:, i, 0], ref, window),
+             "drag_match": bool(abs(Fw[-tail:, i, 0].mean() - ref) <= 0.01 * abs(ref)),
+             "lift_match": bool(abs(Fw[-tail:, i, 1].std() - lift_c) <= 0.1 * max(lift_c, 1e-12) + 1e-9)}
+        res.append(r)
+    valid = [r for r in res if r["cold"] and r["warm"] and r["drag_match"]]
+    ratio = [r["warm"] / r["cold"] for r in valid]
+    # wall time of one case: its steps at the batched rate per case, plus the
+    # network's inference time for the warm start
+    step_case = sc / a.eval_steps / a.test
+    wall = [(r["warm"] * step_case + t_infer / a.test) / (r["cold"] * step_case) for r in valid]
+    ratio_m = [r["mean"] / r["cold"] for r in valid if r["mean"]]
+    out = {"train_cases": int(len(tr_idx)), "test_cases": a.test, "params": net.n_params(), "chans": a.chans,
+           "epochs": a.epochs, "data_s": round(t_data, 1), "train_s": round(t_train, 1),
+           "infer_s_per_case": round(t_infer / a.test, 5), "loss_curve": [round(c, 5) for c in curve],
+           "heldout_rel_rms": round(err, 4), "mean_field_rel_rms": round(err_mean, 4),
+           "cases_scored": len(valid), "cases_evaluated": len(res),
+           "drag_match": int(sum(r["drag_match"] for r in res)), "lift_match": int(sum(r["lift_match"] for r in res)),
+           "median_steps_cold": float(np.median([r["cold"] for r in valid])) if valid else None,
+           "median_steps_warm": float(np.median([r["warm"] for r in valid])) if valid else None,
+           "median_ratio_warm_over_cold": float(np.median(ratio)) if ratio else None,
+           "frac_within_one_third": float(np.mean(np.array(ratio) <= 1.0 / 3.0)) if ratio else None,
+           "median_ratio_meanfield_over_cold": float(np.median(ratio_m)) if ratio_m else None,
+           "median_wall_ratio_warm_over_cold": float(np.median(wall)) if wall else None,
+           "batch_seconds": {"cold": round(sc, 2), "warm": round(sw, 2), "mean_field": round(sm, 2)}}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,51 @@
+/* imexlb_nn.h -- kernels of the warm-start network (no counterpart in the
+ * reference): a small convolutional encoder-decoder that predicts a
+ * developed flow field from the body mask and Reynolds number, trained and
+ * run on the GPU by these kernels.  Tensors are FP32, NCHW, in device memory
+ * the caller owns; every call is asynchronous on `stream` (NULL: the default
+ * stream).  Return 0, or -1 for a bad argument, -2 for a launch error. */
+#ifndef IMEXLB_NN_H
+#define IMEXLB_NN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* out[n,co] = act(bias[co] + sum_ci w[co,ci] (*) in[n,ci]), 3x3, padding 1;
+ * w is [CO][CI][3][3]; bias may be NULL; relu: 1 applies max(0, .). */
+int imexlb_nn_conv3x3(const float *in, const float *w, const float *bias, float *out, int N, int CI, int CO, int H,
+                      int W, int relu, void *stream);
+/* wt[ci][co][2-ky][2-kx] = w[co][ci][ky][kx]: with it, imexlb_nn_conv3x3 of
+ * the output gradient gives the input gradient. */
+int imexlb_nn_flip_weights(const float *w, float *wt, int CI, int CO, void *stream);
+/* dw[co,ci,ky,kx] += sum over n, y, x of dout[n,co,y,x] in[n,ci,y+ky-1,x+kx-1] */
+int imexlb_nn_conv3x3_dw(const float *in, const float *dout, float *dw, int N, int CI, int CO, int H, int W,
+                         void *stream);
+/* db[c] = sum of dout[:, c] */
+int imexlb_nn_bias_grad(const float *dout, float *db, int N, int C, int H, int W, void *stream);
+/* dz = dout where out > 0, else 0 (out: the ReLU output) */
+int imexlb_nn_relu_bwd(const float *out, const float *dout, float *dz, int64_t n, void *stream);
+/* 2x2 average pool, in [N][C][2h][2w] -> out [N][C][h][w], and its gradient */
+int imexlb_nn_pool2(const float *in, float *out, int N, int C, int h, int w, void *stream);
+int imexlb_nn_pool2_bwd(const float *dout, float *din, int N, int C, int h, int w, void *stream);
+/* nearest 2x upsample of in [N][C][h][w] into channels [coff, coff+C) of
+ * out [N][Ctot][2h][2w], and its gradient (din overwritten) */
+int imexlb_nn_up2(const float *in, float *out, int N, int C, int h, int w, int Ctot, int coff, void *stream);
+int imexlb_nn_up2_bwd(const float *dout, float *din, int N, int C, int h, int w, int Ctot, int coff, void *stream);
+/* dst[:, dcoff+c] = src[:, scoff+c] (accumulate: +=), c < C; src has sctot
+ * channels, dst dctot */
+int imexlb_nn_copy_channels(const float *src, int sctot, int scoff, float *dst, int dctot, int dcoff, int N, int C,
+                            int H, int W, int accumulate, void *stream);
+/* mean squared error of n values: grad = 2 (pred - target) / n (grad may be
+ * NULL); *loss += the mean (loss may be NULL) */
+int imexlb_nn_mse(const float *pred, const float *target, float *grad, int64_t n, float *loss, void *stream);
+/* Adam step `step` (1-based) on n weights */
+int imexlb_nn_adam(float *w, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2, float eps,
+                   int step, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IMEXLB_NN_H */

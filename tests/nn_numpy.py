@@ -1,0 +1,117 @@
+"""NumPy restatement of the warm-start network (test infrastructure): the
+same encoder-decoder as imexlb.surrogate.UNet, forward and backward, in
+float64, used to check the HIP kernels' outputs and gradients.  Its own
+gradients are pinned by finite differences (tests/test_nn_numpy.py)."""
+import numpy as np
+
+
+def conv3x3(x, w, b, relu):
+    """x [N][CI][H][W], w [CO][CI][3][3], b [CO]; padding 1."""
+    N, CI, H, W = x.shape
+    xp = np.pad(x, ((0, 0), (0, 0), (1, 1), (1, 1)))
+    out = np.zeros((N, w.shape[0], H, W))
+    for ky in range(3):
+        for kx in range(3):
+            out += np.einsum("oc,nchw->nohw", w[:, :, ky, kx], xp[:, :, ky:ky + H, kx:kx + W])
+    if b is not None:
+        out += b[None, :, None, None]
+    return np.maximum(out, 0.0) if relu else out
+
+
+def conv3x3_backward(x, w, out, dout, relu):
+    """Gradients of conv3x3 (out is its output): (dx, dw, db)."""
+    N, CI, H, W = x.shape
+    dz = dout * (out > 0) if relu else dout
+    xp = np.pad(x, ((0, 0), (0, 0), (1, 1), (1, 1)))
+    dw = np.zeros_like(w)
+    dxp = np.zeros_like(xp)
+    for ky in range(3):
+        for kx in range(3):
+            dw[:, :, ky, kx] = np.einsum("nohw,nchw->oc", dz, xp[:, :, ky:ky + H, kx:kx + W])
+            dxp[:, :, ky:ky + H, kx:kx + W] += np.einsum("oc,nohw->nchw", w[:, :, ky, kx], dz)
+    return dxp[:, :, 1:-1, 1:-1], dw, dz.sum(axis=(0, 2, 3))
+
+
+def pool2(x):
+    N, C, H, W = x.shape
+    return x.reshape(N, C, H // 2, 2, W // 2, 2).mean(axis=(3, 5))
+
+
+def pool2_backward(dout):
+    return np.repeat(np.repeat(dout, 2, axis=2), 2, axis=3) * 0.25
+
+
+def up2(x):
+    return np.repeat(np.repeat(x, 2, axis=2), 2, axis=3)
+
+
+def up2_backward(dout):
+    N, C, H, W = dout.shape
+    return dout.reshape(N, C, H // 2, 2, W // 2, 2).sum(axis=(3, 5))
+
+
+def layer_shapes(cin, cout, chans):
+    """(CO, CI) of every conv in order: encoder pairs, decoder pairs, output."""
+    shapes, prev = [], cin
+    for c in chans:
+        shapes += [(c, prev), (c, c)]
+        prev = c
+    for l in range(len(chans) - 2, -1, -1):
+        shapes += [(chans[l], chans[l + 1] + chans[l]), (chans[l], chans[l])]
+    shapes.append((cout, chans[0]))
+    return shapes
+
+
+def forward(x, params, chans):
+    """params: list of (w, b) in layer_shapes order.  Returns (out, cache)."""
+    acts = []  # (input, output, relu) of every conv, in order
+    it = iter(params)
+
+    def conv(inp, relu=True):
+        w, b = next(it)
+        o = conv3x3(inp, w, b, relu)
+        acts.append((inp, o, relu))
+        return o
+
+    enc, h = [], x
+    for l in range(len(chans)):
+        if l > 0:
+            h = pool2(h)
+        h = conv(conv(h))
+        enc.append(h)
+    d = enc[-1]
+    for l in range(len(chans) - 2, -1, -1):
+        cat = np.concatenate([up2(d), enc[l]], axis=1)
+        d = conv(conv(cat))
+    out = conv(d, relu=False)
+    return out, (acts, enc)
+
+
+def backward(dout, params, chans, cache):
+    """Gradients [(dw, db)] in layer order for the loss gradient dout."""
+    acts, enc = cache
+    L = len(chans)
+    grads = [None] * len(params)
+    k = len(params) - 1
+
+    def conv_b(g):
+        nonlocal k
+        inp, o, relu = acts[k]
+        dx, dw, db = conv3x3_backward(inp, params[k][0], o, g, relu)
+        grads[k] = (dw, db)
+        k -= 1
+        return dx
+
+    g = conv_b(dout)
+    denc = [np.zeros_like(e) for e in enc]
+    for l in range(0, L - 1):
+        g = conv_b(conv_b(g))  # the decoder pair at level l, innermost last
+        cu = chans[l + 1]
+        denc[l] += g[:, cu:]
+        g = up2_backward(g[:, :cu])
+    g = g + denc[L - 1]
+    for l in range(L - 1, -1, -1):
+        g = conv_b(conv_b(g))
+        if l > 0:
+            g = pool2_backward(g) + denc[l - 1]
+    return grads

@@ -1,12 +1,14 @@
 """Learned warm start for D2Q9 cylinder/ellipse cases.
 
-    python -m imexlb.surrogate [--train 2000] [--test 200] [--epochs 15]
+    python -m imexlb.surrogate [--train 2000] [--test 200] [--epochs 20]
 
 1. Cases: 512x128 channel, a body of the cylinder's area (radius Ny/10) at
    x = 0.2 Nx, a circle or an ellipse of axis ratio U[0.5, 2] turned by
    U[0, 90] degrees; Re log-uniform in [40, 300]; inlet velocity 0.05.
 2. Data: every case run by the batched solver (FP16 storage) to a developed
-   wake; the target is its (p, u, v) field averaged over the last snapshots.
+   wake; the target is its (p, u, v) field at one phase of the shedding
+   cycle (the lift's upward zero crossing), so the network predicts a state
+   that already sheds at the developed amplitude.
 3. Network: a 4-level convolutional encoder-decoder (3x3 convolutions with
    ReLU, 2x2 average pooling, nearest upsampling, skip connections), FP32,
    trained with Adam by the HIP kernels of include/imexlb_nn.h.  Input:
@@ -271,29 +273,43 @@ def inputs(cases, masks):
     return x
 
 
-def developed_fields(cases, steps, avg_snapshots, avg_every, r_vec, precision, log=None):
-    """Run every case (batched) for `steps`; return (M, 3, NY, NX) float32 of
-    (p / uu^2, u / uu, v / uu) averaged over the last snapshots, and the
-    final forces."""
+def developed_fields(cases, steps, r_vec, precision, probe_every=20, max_probe_steps=6000, log=None):
+    """Run every case (batched) for `steps`, then capture each case's field
+    at the same phase of its shedding cycle: the first probe (every
+    `probe_every` steps) at which its lift turns from negative to
+    non-negative.  Cases that do not shed within `max_probe_steps` are taken
+    at the end.  Returns (M, 3, NY, NX) float32 of (p / uu^2, u / uu,
+    v / uu) and the capture step of every case."""
     b = _batch(cases, precision)
     b.init(r_vec)
-    first = steps - (avg_snapshots - 1) * avg_every
-    b.advance(first)
-    acc = np.zeros((len(cases), 3, NY, NX))
-    for s in range(avg_snapshots):
-        if s:
-            b.advance(avg_every)
-        for i in range(len(cases)):
-            p, u = b.download_fields(i)
-            n = (NX + 1) * (NY + 1)
-            acc[i, 0] += p.reshape(NY + 1, NX + 1)[:NY, :NX] / UU ** 2
-            acc[i, 1] += u[:n].reshape(NY + 1, NX + 1)[:NY, :NX] / UU
-            acc[i, 2] += u[n:].reshape(NY + 1, NX + 1)[:NY, :NX] / UU
-        if log:
-            log(f"[surrogate] snapshot {s + 1}/{avg_snapshots}")
-    F = b.forces()
+    b.advance(steps)
+    M = len(cases)
+    out = np.zeros((M, 3, NY, NX), np.float32)
+    when = np.full(M, -1)
+    n = (NX + 1) * (NY + 1)
+
+    def grab(i, t):
+        p, u = b.download_fields(i)
+        out[i, 0] = p.reshape(NY + 1, NX + 1)[:NY, :NX] / UU ** 2
+        out[i, 1] = u[:n].reshape(NY + 1, NX + 1)[:NY, :NX] / UU
+        out[i, 2] = u[n:].reshape(NY + 1, NX + 1)[:NY, :NX] / UU
+        when[i] = t
+
+    prev = b.forces()[:, 1]
+    t = steps
+    while t < steps + max_probe_steps and (when < 0).any():
+        b.advance(probe_every)
+        t += probe_every
+        lift = b.forces()[:, 1]
+        for i in np.nonzero((when < 0) & (prev < 0) & (lift >= 0))[0]:
+            grab(i, t)
+        prev = lift
+    if log:
+        log(f"[surrogate] {int((when >= 0).sum())} of {M} cases captured at the lift's upward zero crossing")
+    for i in np.nonzero(when < 0)[0]:
+        grab(i, t)
     b.close()
-    return (acc / avg_snapshots).astype(np.float32), F
+    return out, when
 
 
 def to_lattice_fields(pred, r_vec):
@@ -352,13 +368,12 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=2000)
     ap.add_argument("--test", type=int, default=200)
-    ap.add_argument("--data-steps", type=int, default=16000)
-    ap.add_argument("--snapshots", type=int, default=8)
-    ap.add_argument("--snapshot-every", type=int, default=500)
-    ap.add_argument("--epochs", type=int, default=15)
+    ap.add_argument("--data-steps", type=int, default=24000)
+    ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--chans", default="16,32,64,128")
+    ap.add_argument("--lr", type=float, default=5e-4)
+    ap.add_argument("--chans", default="32,64,128,256")
+    ap.add_argument("--infer-batch", type=int, default=40)
     ap.add_argument("--eval-steps", type=int, default=40000)
     ap.add_argument("--seed", type=int, default=2024)
     a = ap.parse_args(argv)
@@ -372,7 +387,7 @@ def main(argv=None):
     X = inputs(cases, masks)
     log(f"[surrogate] data: {len(cases)} cases, {a.data_steps} steps")
     t0 = time.perf_counter()
-    Y, _ = developed_fields(cases, a.data_steps, a.snapshots, a.snapshot_every, r_vec, prec, log)
+    Y, _ = developed_fields(cases, a.data_steps, r_vec, prec, log=log)
     t_data = time.perf_counter() - t0
     finite = np.isfinite(Y).all(axis=(1, 2, 3))
     tr_idx = np.nonzero(finite[:a.train])[0]
@@ -406,20 +421,51 @@ def main(argv=None):
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
 
-    # predictions for the held-out cases
+    # predictions for the held-out cases (a network of the same weights with
+    # a larger batch for inference)
     te_idx = np.arange(a.train, a.train + a.test)
     preds = np.zeros((a.test, 3, NY, NX), np.float32)
+    NB = min(a.infer_batch, a.test)
+    infer = UNet(4, 3, net.chans, NB, NY, NX, params=net.params_numpy())
+    xi = torch.empty((NB, 4, NY, NX), device="cuda")
+    infer.forward(xi.zero_())  # first launch outside the timing
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for s in range(0, a.test, a.batch):
-        sel = te_idx[s:s + a.batch]
-        sel = np.concatenate([sel, np.full(a.batch - len(sel), sel[-1])])
-        xb.copy_(Xd.index_select(0, torch.tensor(sel, device="cuda")))
-        out = net.forward(xb)
-        k = min(a.batch, a.test - s)
+    for s in range(0, a.test, NB):
+        sel = te_idx[s:s + NB]
+        sel = np.concatenate([sel, np.full(NB - len(sel), sel[-1])])
+        xi.copy_(Xd.index_select(0, torch.tensor(sel, device="cuda")))
+        out = infer.forward(xi)
+        k = min(NB, a.test - s)
         preds[s:s + k] = out[:k].cpu().numpy()
     torch.cuda.synchronize()
     t_infer = time.perf_counter() - t0
+    # one case alone: single-lattice step time and a batch-1 inference
+    one = UNet(4, 3, net.chans, 1, NY, NX, params=net.params_numpy())
+    x1 = torch.empty((1, 4, NY, NX), device="cuda")
+    x1.copy_(Xd[te_idx[0]:te_idx[0] + 1])
+    one.forward(x1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        one.forward(x1)
+    torch.cuda.synchronize()
+    t_infer_one = (time.perf_counter() - t0) / 5
+    from .lattice import Lattice
+    dk, asp, ang = test[0]
+    dd = derive(dk)
+    lat = Lattice(2, NX, NY, 0, device=0, precision=prec)
+    lat.set_params(dk.ome, dk.rho0, dd.uu)
+    lat.set_geometry(dd.pos, dd.radius, asp, ang)
+    lat.init_up(r_vec)
+    lat.init_pdf()
+    lat.advance(64)
+    lat.synchronize()
+    t0 = time.perf_counter()
+    lat.advance(4096)
+    lat.synchronize()
+    t_step_one = (time.perf_counter() - t0) / 4096
+    lat.close()
     held = Y[te_idx]
     ok = finite[te_idx]
     err = float(np.sqrt(np.mean((preds[ok] - held[ok]) ** 2)) / np.sqrt(np.mean(held[ok] ** 2)))
@@ -447,16 +493,20 @@ def main(argv=None):
              "warm": convergence_step(tw, Fw[:, i, 0], ref, window),
              "mean": convergence_step(tm, Fm[:, i, 0], ref, window),
              "drag_match": bool(abs(Fw[-tail:, i, 0].mean() - ref) <= 0.01 * abs(ref)),
-             "lift_match": bool(abs(Fw[-tail:, i, 1].std() - lift_c) <= 0.1 * max(lift_c, 1e-12) + 1e-9)}
+             # lift amplitude within 10 %, or within 1 % of the drag for cases that barely shed
+             "lift_match": bool(abs(Fw[-tail:, i, 1].std() - lift_c) <= max(0.1 * lift_c, 0.01 * abs(ref)))}
         res.append(r)
-    valid = [r for r in res if r["cold"] and r["warm"] and r["drag_match"]]
+    valid = [r for r in res if r["cold"] and r["warm"] and r["drag_match"] and r["lift_match"]]
     ratio = [r["warm"] / r["cold"] for r in valid]
-    # wall time of one case: its steps at the batched rate per case, plus the
-    # network's inference time for the warm start
+    # wall time of one case: its steps at the batched rate per case plus its
+    # share of the batched inference; and run alone: single-lattice steps
+    # plus a batch-1 inference
     step_case = sc / a.eval_steps / a.test
     wall = [(r["warm"] * step_case + t_infer / a.test) / (r["cold"] * step_case) for r in valid]
+    wall_one = [(r["warm"] * t_step_one + t_infer_one) / (r["cold"] * t_step_one) for r in valid]
     ratio_m = [r["mean"] / r["cold"] for r in valid if r["mean"]]
     out = {"train_cases": int(len(tr_idx)), "test_cases": a.test, "params": net.n_params(), "chans": a.chans,
+           "infer_batch": NB,
            "epochs": a.epochs, "data_s": round(t_data, 1), "train_s": round(t_train, 1),
            "infer_s_per_case": round(t_infer / a.test, 5), "loss_curve": [round(c, 5) for c in curve],
            "heldout_rel_rms": round(err, 4), "mean_field_rel_rms": round(err_mean, 4),
@@ -468,6 +518,9 @@ def main(argv=None):
            "frac_within_one_third": float(np.mean(np.array(ratio) <= 1.0 / 3.0)) if ratio else None,
            "median_ratio_meanfield_over_cold": float(np.median(ratio_m)) if ratio_m else None,
            "median_wall_ratio_warm_over_cold": float(np.median(wall)) if wall else None,
+           "frac_wall_within_one_third": float(np.mean(np.array(wall) <= 1.0 / 3.0)) if wall else None,
+           "median_wall_ratio_single_case": float(np.median(wall_one)) if wall_one else None,
+           "single_case": {"step_ms": round(t_step_one * 1e3, 5), "infer_ms": round(t_infer_one * 1e3, 3)},
            "batch_seconds": {"cold": round(sc, 2), "warm": round(sw, 2), "mean_field": round(sm, 2)}}
     print(json.dumps(out), flush=True)
 

@@ -368,6 +368,37 @@ int imexlb_batch_init_fields(imexlb_batch *b, const double *p, const double *u)
     return batch_start(b, nullptr);
 }
 
+int imexlb_batch_init_fields_f32(imexlb_batch *b, const float *fields, int H, int W, double p_scale,
+                                 double u_scale, const double *r_vec)
+{
+    CHECK_BATCH(b);
+    if (!fields || H <= 0 || W <= 0) return fail(IMEXLB_E_ARG, "fields");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (r_vec) HIP_TRY(hipMemcpyAsync(b->rv, r_vec, 100 * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    const int64_t per = 3 * int64_t(H) * W;
+    for (int i = 0; i < b->ncase; ++i) {
+        hipError_t e = launch_init_given(b->case_args(i), r_vec ? b->rv : nullptr, fields + per * i, H, W, p_scale,
+                                         u_scale, b->stream);
+        if (e != hipSuccess) return fail(IMEXLB_E_HIP, std::string("batch fields: ") + hipGetErrorString(e));
+    }
+    return batch_start(b, nullptr);
+}
+
+int imexlb_batch_body_mask_f32(imexlb_batch *b, float *out, int H, int W, int64_t stride)
+{
+    CHECK_BATCH(b);
+    if (!out || H <= 0 || W <= 0 || stride < int64_t(H) * W) return fail(IMEXLB_E_ARG, "mask size");
+    for (int i = 0; i < b->ncase; ++i) {
+        const StepArgs a = b->case_args(i);
+        if (H > a.kp.ny + 1 || W > a.kp.nx + 1) return fail(IMEXLB_E_ARG, "mask larger than a case");
+    }
+    for (int i = 0; i < b->ncase; ++i) {
+        hipError_t e = launch_body_mask(b->case_args(i), out + stride * i, H, W, b->stream);
+        if (e != hipSuccess) return fail(IMEXLB_E_HIP, std::string("body mask: ") + hipGetErrorString(e));
+    }
+    return IMEXLB_OK;
+}
+
 int imexlb_batch_set_config(imexlb_batch *b, int block, int cells_per_thread, int persistent)
 {
     CHECK_BATCH(b);

@@ -499,6 +499,37 @@ int imexlb_init_up(imexlb_ctx *c, const double *r_vec)
 }
 
 
+int imexlb_init_fields_f32(imexlb_ctx *c, const float *fields, int H, int W, double p_scale, double u_scale,
+                           const double *r_vec)
+{
+    CHECK_DEV(c);
+    if (c->dim != 2) return fail(IMEXLB_E_ARG, "FP32 field stacks are for 2D lattices");
+    if (!fields || H <= 0 || W <= 0) return fail(IMEXLB_E_ARG, "fields");
+    if (!c->geom_set) return fail(IMEXLB_E_STATE, "SetGeometry before setting fields");
+    if (r_vec) HIP_TRY(hipMemcpyAsync(c->rv, r_vec, 100 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    StepArgs a = c->args();
+    int rc = c->launch(2, [&] {
+        return launch_init_given(a, r_vec ? c->rv : nullptr, fields, H, W, p_scale, u_scale, c->stream);
+    });
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));  // r_vec is the caller's
+    c->have_pu = true;
+    c->have_f = c->have_fb = false;
+    c->lazy_src = -1;
+    c->stage = S_NONE;
+    return IMEXLB_OK;
+}
+
+int imexlb_body_mask_f32(imexlb_ctx *c, float *out, int H, int W)
+{
+    CHECK_DEV(c);
+    if (c->dim != 2) return fail(IMEXLB_E_ARG, "body masks are for 2D lattices");
+    if (!out || H <= 0 || W <= 0 || H > c->kp.ny + 1 || W > c->kp.nx + 1) return fail(IMEXLB_E_ARG, "mask size");
+    if (!c->geom_set) return fail(IMEXLB_E_STATE, "SetGeometry before the body mask");
+    StepArgs a = c->args();
+    return c->launch(2, [&] { return launch_body_mask(a, out, H, W, c->stream); });
+}
+
 int imexlb_upload_fields(imexlb_ctx *c, const double *p, const double *u, int layout)
 {
     CHECK_DEV(c);

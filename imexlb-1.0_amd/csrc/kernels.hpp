@@ -113,6 +113,12 @@ hipError_t launch_step(int dim, int prec, int mode, const StepArgs &a, hipStream
 hipError_t launch_collide(int dim, int prec, const StepArgs &a, hipStream_t s);
 hipError_t launch_init_pdf(int dim, int prec, const StepArgs &a, hipStream_t s);
 hipError_t launch_init_up(int dim, const StepArgs &a, const double *rv, hipStream_t s);
+// 2D fields from an FP32 device stack [3][H][W] (p, u_x, u_y), scaled by
+// ps / us, u_x times InitUP's perturbation when rv is given, 0 on bodies
+hipError_t launch_init_given(const StepArgs &a, const double *rv, const float *f, int H, int W, double ps, double us,
+                             hipStream_t s);
+// out[y W + x] = 1 on body nodes, 0 on fluid nodes (2D, y < H, x < W)
+hipError_t launch_body_mask(const StepArgs &a, float *out, int H, int W, hipStream_t s);
 // Boundary fill; the slot tables hold int32 (idx32) or int64 entries.
 hipError_t launch_fill(int prec, void *buf, const void *prev, const void *dslot, const void *sslot, bool idx32,
                        int64_t n, hipStream_t s);

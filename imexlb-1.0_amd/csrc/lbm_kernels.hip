@@ -639,6 +639,39 @@ __global__ __launch_bounds__(256) void k_init_up(double *__restrict__ p, double 
     }
 }
 
+// Fields of a 2D lattice from an FP32 stack f[3][H][W] (p, u_x, u_y at node
+// rows y < H and columns x < W; nodes beyond take the last row and column):
+// p = ps f0, u_x = us f1 r, u_y = us f2, with r InitUP's perturbation
+// factor when rv is given (1 otherwise), and 0 on body nodes.
+__global__ __launch_bounds__(256) void k_init_given(double *__restrict__ p, double *__restrict__ ux,
+                                                    double *__restrict__ uy, const uint8_t *__restrict__ geo,
+                                                    const double *__restrict__ rv, KParams kp,
+                                                    const float *__restrict__ f, int H, int W, double ps, double us)
+{
+    int x, y, z;
+    uint32_t c;
+    if (!decode<2>(kp, blockIdx.x * blockDim.x + threadIdx.x, x, y, z, c)) return;
+    const int64_t rid = int64_t(x + 3) + kp.rdy * (y + 3);
+    const double r = rv ? 1.0 + (rv[rid % 100] - 0.5) * 2.0 * 0.2 : 1.0;
+    const double g = double(geo[c] & G_FLUID);
+    const int64_t k = int64_t(y < H ? y : H - 1) * W + (x < W ? x : W - 1);
+    const int64_t hw = int64_t(H) * W;
+    p[c] = ps * double(f[k]) * g;
+    ux[c] = us * double(f[hw + k]) * r * g;
+    uy[c] = us * double(f[2 * hw + k]) * g;
+}
+
+// out[y W + x] = 1 on body nodes, 0 on fluid nodes, for nodes y < H, x < W
+// of a 2D lattice
+__global__ __launch_bounds__(256) void k_body_mask(float *__restrict__ out, const uint8_t *__restrict__ geo,
+                                                   KParams kp, int H, int W)
+{
+    int x, y, z;
+    uint32_t c;
+    if (!decode<2>(kp, blockIdx.x * blockDim.x + threadIdx.x, x, y, z, c)) return;
+    if (x < W && y < H) out[int64_t(y) * W + x] = (geo[c] & G_FLUID) ? 0.0f : 1.0f;
+}
+
 // Boundary fill of this step's buffer: dst slot <- this step's fluid slot
 // (s >= 0), 0 (s == -1), or a body-cell slot of the previous step's buffer
 // (s <= -2, slot -s-2).  Sources are never destinations, so the order of
@@ -830,6 +863,20 @@ hipError_t launch_init_up(int dim, const StepArgs &a, const double *rv, hipStrea
     else
         hipLaunchKernelGGL((k_init_up<3>), dim3(nblocks(a.kp.N)), dim3(256), 0, s, a.p, a.ux, a.uy, a.uz, a.geo, rv,
                            a.kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_given(const StepArgs &a, const double *rv, const float *f, int H, int W, double ps, double us,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(k_init_given, dim3(nblocks(a.kp.N)), dim3(256), 0, s, a.p, a.ux, a.uy, a.geo, rv, a.kp, f, H,
+                       W, ps, us);
+    return hipGetLastError();
+}
+
+hipError_t launch_body_mask(const StepArgs &a, float *out, int H, int W, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_body_mask, dim3(nblocks(a.kp.N)), dim3(256), 0, s, out, a.geo, a.kp, H, W);
     return hipGetLastError();
 }
 

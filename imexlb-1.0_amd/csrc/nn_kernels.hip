@@ -10,6 +10,18 @@
 // data gradient is the same kernel on the flipped, transposed weights; the
 // weight gradient gives each thread one (co, ci) pair and its 9 taps over a
 // tile, summed over the batch, then added to dW with atomics.
+//
+// Inference in BF16: activations NHWC bf16, weights packed
+// [CI/8][tap][CO][8 ci] bf16, products on the matrix cores
+// (v_mfma_f32_32x32x16_bf16, FP32 accumulation), bias, ReLU and the
+// rounding to bf16 (round to nearest even) fused into the store.  One block
+// computes 8x16 pixels for 32 or 64 output channels, one 32-pixel row pair
+// per wave; the input tile with its halo and the block's weights go through
+// LDS 16 or 32 channels at a time, stored in 16-byte groups of 8 channels so
+// that every lane's operand is one 128-bit LDS read and the 32 lanes of a
+// half-wave read consecutive groups.  Skip connections cost no copy: an
+// encoder convolution writes into its channel slice of the decoder's
+// concatenated input (output channel stride and offset).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -252,6 +264,169 @@ __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float
     w[i] -= lr * (mi / c1) / (__builtin_sqrtf(vi / c2) + eps);
 }
 
+// ------------------------------------------------------------ BF16 inference
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// FP32 -> bf16, round to nearest even (NaN kept quiet)
+__device__ __forceinline__ uint16_t to_bf16(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return uint16_t((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+
+__device__ __forceinline__ float from_bf16(uint32_t h) { return __uint_as_float(h << 16); }
+
+constexpr int PR = 8, PC = 16;  // pixel tile of the BF16 convolution: 8 rows x 16 columns
+
+// KC input channels per LDS chunk (16 or 32), BN output channels per block
+// (32 or 64).  in: [N][H][W][CIP] bf16 as 16-byte groups; wp: [CIP/8][9][COP]
+// groups of 8 input channels.  Output channel co < CO of pixel (n, y, x):
+// bf16 at out[((n H + y) W + x) ostride + ooff + co], or with f32_nchw FP32
+// at out[((n CO + co) H + y) W + x].
+template <int KC, int BN>
+__global__ __launch_bounds__(256) void k_conv3_bf16(const uint4 *__restrict__ in, const uint4 *__restrict__ wp,
+                                                    const float *__restrict__ bias, void *__restrict__ out, int H,
+                                                    int W, int CIP, int CO, int COP, int ostride, int ooff, int relu,
+                                                    int f32_nchw)
+{
+    constexpr int G = KC / 8;  // channel groups per chunk
+    constexpr int NT = BN / 32;
+    __shared__ uint4 s_in[G][PR + 2][PC + 2];
+    __shared__ uint4 s_w[9][G][BN];
+    const int lane = int(threadIdx.x) % 64, wave = int(threadIdx.x) / 64;
+    const int ntx = (W + PC - 1) / PC;
+    const int x0 = int(blockIdx.x) % ntx * PC, y0 = int(blockIdx.x) / ntx * PR;
+    const int co0 = int(blockIdx.y) * BN, n = int(blockIdx.z);
+    const int cg = CIP / 8;
+    const int m = lane % 32, hk = lane / 32;        // operand row (pixel / channel) and K half
+    const int pr = wave * 2 + m / 16, pc = m % 16;  // this lane's pixel in the tile
+    f32x16 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int j = 0; j < 16; ++j) acc[t][j] = 0.0f;
+    const int64_t img = int64_t(n) * H;
+    for (int c0 = 0; c0 < CIP; c0 += KC) {
+        for (int e = int(threadIdx.x); e < (PR + 2) * (PC + 2) * G; e += 256) {
+            const int g = e % G, p = e / G;
+            const int r = p / (PC + 2), c = p % (PC + 2);
+            const int yy = y0 + r - 1, xx = x0 + c - 1;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W) v = in[((img + yy) * W + xx) * cg + c0 / 8 + g];
+            s_in[g][r][c] = v;
+        }
+        for (int e = int(threadIdx.x); e < 9 * G * BN; e += 256) {
+            const int co = e % BN, t = (e / BN) % 9, g = e / (9 * BN);
+            s_w[t][g][co] = wp[(int64_t(c0 / 8 + g) * 9 + t) * COP + co0 + co];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 9; ++t)
+#pragma unroll
+            for (int s = 0; s < KC / 16; ++s) {
+                const bf16x8 a = __builtin_bit_cast(bf16x8, s_in[2 * s + hk][pr + t / 3][pc + t % 3]);
+#pragma unroll
+                for (int q = 0; q < NT; ++q) {
+                    const bf16x8 b = __builtin_bit_cast(bf16x8, s_w[t][2 * s + hk][q * 32 + m]);
+                    acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[q], 0, 0, 0);
+                }
+            }
+        __syncthreads();
+    }
+    // lane holds D[pixel 8 (j/4) + 4 hk + j%4][channel m] of each 32x32 tile
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+        const int co = co0 + q * 32 + m;
+        if (co >= CO) continue;
+        const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int px = 8 * (j / 4) + 4 * hk + j % 4;
+            const int y = y0 + wave * 2 + px / 16, x = x0 + px % 16;
+            if (y >= H || x >= W) continue;
+            float v = acc[q][j] + bv;
+            if (relu) v = v > 0.0f ? v : 0.0f;
+            if (f32_nchw)
+                static_cast<float *>(out)[((int64_t(n) * CO + co) * H + y) * W + x] = v;
+            else
+                static_cast<uint16_t *>(out)[((img + y) * W + x) * ostride + ooff + co] = to_bf16(v);
+        }
+    }
+}
+
+// x [N][C][H][W] FP32 -> y [N][H][W][CP] bf16, channels C..CP-1 zero
+__global__ void k_pack_in(const float *__restrict__ x, uint16_t *__restrict__ y, int C, int CP, int H, int W,
+                          int64_t total)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int c = int(i % CP);
+    const int64_t p = i / CP;
+    const int64_t hw = int64_t(H) * W;
+    const int64_t n = p / hw, r = p % hw;
+    y[i] = c < C ? to_bf16(x[(n * C + c) * hw + r]) : uint16_t(0);
+}
+
+// w [CO][CI][3][3] FP32 -> wp [CIP/8][9][COP][8] bf16, zero outside CO x CI
+__global__ void k_pack_w(const float *__restrict__ w, uint16_t *__restrict__ wp, int CI, int CO, int COP,
+                         int64_t total)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int j = int(i % 8), co = int((i / 8) % COP), t = int((i / (8 * int64_t(COP))) % 9);
+    const int ci = int(i / (72 * int64_t(COP))) * 8 + j;
+    wp[i] = (co < CO && ci < CI) ? to_bf16(w[(int64_t(co) * CI + ci) * 9 + t]) : uint16_t(0);
+}
+
+// 2x2 average pool of channels [coff, coff + 8 C8) of in [N][2h][2w][8 cs8]
+// into out [N][h][w][8 C8]; one thread per 8 channels
+__global__ void k_pool2_bf16(const uint4 *__restrict__ in, uint4 *__restrict__ out, int h, int w, int cs8, int coff8,
+                             int C8, int64_t total)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int g = int(i % C8);
+    const int64_t p = i / C8;
+    const int x = int(p % w), y = int((p / w) % h);
+    const int64_t n = p / (int64_t(w) * h);
+    const int64_t b = ((n * 2 * h + 2 * y) * 2 * w + 2 * x) * cs8 + coff8 + g;
+    const uint4 v[4] = {in[b], in[b + cs8], in[b + 2 * int64_t(w) * cs8], in[b + (2 * int64_t(w) + 1) * cs8]};
+    uint32_t o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        uint32_t r = 0;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            float s[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t word = k == 0 ? v[q].x : k == 1 ? v[q].y : k == 2 ? v[q].z : v[q].w;
+                s[q] = from_bf16(half ? word >> 16 : word & 0xffffu);
+            }
+            r |= uint32_t(to_bf16(0.25f * ((s[0] + s[1]) + (s[2] + s[3])))) << (16 * half);
+        }
+        o[k] = r;
+    }
+    out[i] = make_uint4(o[0], o[1], o[2], o[3]);
+}
+
+// nearest 2x upsample of in [N][h][w][8 C8] into channels [coff, coff + 8 C8)
+// of out [N][2h][2w][8 ct8]
+__global__ void k_up2_bf16(const uint4 *__restrict__ in, uint4 *__restrict__ out, int h, int w, int ct8, int coff8,
+                           int C8, int64_t total)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int g = int(i % C8);
+    const int64_t p = i / C8;
+    const int X = int(p % (2 * w)), Y = int((p / (2 * w)) % (2 * h));
+    const int64_t n = p / (int64_t(4) * w * h);
+    out[p * ct8 + coff8 + g] = in[((n * h + Y / 2) * w + X / 2) * C8 + g];
+}
+
 inline unsigned blocks(int64_t n) { return unsigned((n + 255) / 256); }
 
 }  // namespace imexlb_nn
@@ -358,6 +533,79 @@ int imexlb_nn_adam(float *w, const float *g, float *m, float *v, int64_t n, floa
     c1 = 1.0f - p1;
     c2 = 1.0f - p2;
     NN_LAUNCH(k_adam, dim3(blocks(n)), dim3(256), 0, (hipStream_t)stream, w, g, m, v, n, lr, b1, b2, eps, c1, c2);
+}
+
+int imexlb_nn_pack_input_bf16(const float *x, uint16_t *y, int N, int C, int CP, int H, int W, void *stream)
+{
+    if (!x || !y || N <= 0 || C <= 0 || CP < C || H <= 0 || W <= 0) return -1;
+    const int64_t total = int64_t(N) * H * W * CP;
+    NN_LAUNCH(k_pack_in, dim3(blocks(total)), dim3(256), 0, (hipStream_t)stream, x, y, C, CP, H, W, total);
+}
+
+int imexlb_nn_pack_weights_bf16(const float *w, uint16_t *wp, int CI, int CO, int CIP, int COP, void *stream)
+{
+    if (!w || !wp || CI <= 0 || CO <= 0 || CIP < CI || CIP % 8 || COP < CO) return -1;
+    const int64_t total = int64_t(CIP) * 9 * COP;
+    NN_LAUNCH(k_pack_w, dim3(blocks(total)), dim3(256), 0, (hipStream_t)stream, w, wp, CI, CO, COP, total);
+}
+
+int imexlb_nn_conv3x3_bf16(const uint16_t *in, const uint16_t *wp, const float *bias, void *out, int N, int CIP,
+                           int CO, int COP, int H, int W, int ostride, int ooff, int relu, int f32_nchw, void *stream)
+{
+    if (!in || !wp || !out || N <= 0 || N > 65535 || H <= 0 || W <= 0 || CIP <= 0 || CIP % 16 || CO <= 0 ||
+        COP < CO || COP % 32)
+        return -1;
+    if (!f32_nchw && (ooff < 0 || ooff + CO > ostride)) return -1;
+    if ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(wp)) & 15) return -1;
+    const int64_t tiles = int64_t((W + PC - 1) / PC) * ((H + PR - 1) / PR);
+    if (tiles > 0x7fffffff) return -1;
+    const bool wide = COP % 64 == 0 && CO > 32;
+    const dim3 grid(unsigned(tiles), unsigned(COP / (wide ? 64 : 32)), unsigned(N));
+    const auto *a = reinterpret_cast<const uint4 *>(in);
+    const auto *b = reinterpret_cast<const uint4 *>(wp);
+    hipStream_t s = (hipStream_t)stream;
+    if (CIP % 32 == 0) {
+        if (wide)
+            hipLaunchKernelGGL((k_conv3_bf16<32, 64>), grid, dim3(256), 0, s, a, b, bias, out, H, W, CIP, CO, COP,
+                               ostride, ooff, relu, f32_nchw);
+        else
+            hipLaunchKernelGGL((k_conv3_bf16<32, 32>), grid, dim3(256), 0, s, a, b, bias, out, H, W, CIP, CO, COP,
+                               ostride, ooff, relu, f32_nchw);
+    } else {
+        if (wide)
+            hipLaunchKernelGGL((k_conv3_bf16<16, 64>), grid, dim3(256), 0, s, a, b, bias, out, H, W, CIP, CO, COP,
+                               ostride, ooff, relu, f32_nchw);
+        else
+            hipLaunchKernelGGL((k_conv3_bf16<16, 32>), grid, dim3(256), 0, s, a, b, bias, out, H, W, CIP, CO, COP,
+                               ostride, ooff, relu, f32_nchw);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int imexlb_nn_pool2_bf16(const uint16_t *in, uint16_t *out, int N, int C, int h, int w, int cstride, int coff,
+                         void *stream)
+{
+    if (!in || !out || N <= 0 || C <= 0 || h <= 0 || w <= 0 || C % 8 || cstride % 8 || coff % 8 || coff < 0 ||
+        coff + C > cstride)
+        return -1;
+    if ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) return -1;
+    const int64_t total = int64_t(N) * h * w * (C / 8);
+    NN_LAUNCH(k_pool2_bf16, dim3(blocks(total)), dim3(256), 0, (hipStream_t)stream,
+              reinterpret_cast<const uint4 *>(in), reinterpret_cast<uint4 *>(out), h, w, cstride / 8, coff / 8, C / 8,
+              total);
+}
+
+int imexlb_nn_up2_bf16(const uint16_t *in, uint16_t *out, int N, int C, int h, int w, int Ctot, int coff,
+                       void *stream)
+{
+    if (!in || !out || N <= 0 || C <= 0 || h <= 0 || w <= 0 || C % 8 || Ctot % 8 || coff % 8 || coff < 0 ||
+        coff + C > Ctot)
+        return -1;
+    if ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) return -1;
+    const int64_t total = int64_t(N) * 4 * h * w * (C / 8);
+    NN_LAUNCH(k_up2_bf16, dim3(blocks(total)), dim3(256), 0, (hipStream_t)stream,
+              reinterpret_cast<const uint4 *>(in), reinterpret_cast<uint4 *>(out), h, w, Ctot / 8, coff / 8, C / 8,
+              total);
 }
 
 }  // extern "C"

@@ -39,6 +39,8 @@ SIGNATURES = {
     "imexlb_bc_table": (C.c_int, [_P, _I64P, _I64P, C.c_int64, _I64P]),
     "imexlb_init_up": (C.c_int, [_P, _DP]),
     "imexlb_upload_fields": (C.c_int, [_P, _DP, _DP, C.c_int]),
+    "imexlb_init_fields_f32": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_double, C.c_double, _DP]),
+    "imexlb_body_mask_f32": (C.c_int, [_P, _P, C.c_int, C.c_int]),
     "imexlb_init_pdf": (C.c_int, [_P]),
     "imexlb_collision": (C.c_int, [_P]),
     "imexlb_boundary_condition": (C.c_int, [_P]),
@@ -59,6 +61,8 @@ SIGNATURES = {
     "imexlb_batch_destroy": (C.c_int, [_P]),
     "imexlb_batch_init": (C.c_int, [_P, _DP]),
     "imexlb_batch_init_fields": (C.c_int, [_P, _DP, _DP]),
+    "imexlb_batch_init_fields_f32": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_double, C.c_double, _DP]),
+    "imexlb_batch_body_mask_f32": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int64]),
     "imexlb_batch_advance": (C.c_int, [_P, C.c_int64]),
     "imexlb_batch_set_config": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
     "imexlb_batch_tune": (C.c_int, [_P, C.c_int, _I32P, _DP]),
@@ -81,6 +85,12 @@ SIGNATURES = {
     "imexlb_nn_mse": (C.c_int, [_P, _P, _P, C.c_int64, _P, _P]),
     "imexlb_nn_adam": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
                                  _P]),
+    "imexlb_nn_pack_input_bf16": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
+    "imexlb_nn_pack_weights_bf16": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
+    "imexlb_nn_conv3x3_bf16": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                         C.c_int, C.c_int, C.c_int, C.c_int, _P]),
+    "imexlb_nn_pool2_bf16": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
+    "imexlb_nn_up2_bf16": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
 }
 
 
@@ -99,6 +109,13 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"libimexlb.so not built at {LIB_PATH}: run __graft_entry__.build()")
+        # PyTorch ships its own HIP runtime.  Loaded after this library has
+        # started the system one, it finds no GPU; loaded first, both work
+        # in one process.  So torch, when installed, is imported first.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

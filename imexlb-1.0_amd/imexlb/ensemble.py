@@ -77,6 +77,25 @@ class Batch:
         check(lib().imexlb_batch_init_fields(self._h, p.ctypes.data_as(_DP), u.ctypes.data_as(_DP)),
               "imexlb_batch_init_fields")
 
+    def init_fields_f32(self, fields, p_scale, u_scale, r_vec=None):
+        """Warm start every case from a device FP32 tensor (ncase, 3, H, W)
+        of p, u_x, u_y (see Lattice.init_fields_f32)."""
+        if fields.dim() != 4 or fields.shape[0] != self.ncase or fields.shape[1] != 3 or not fields.is_contiguous():
+            raise ValueError("fields: contiguous (ncase, 3, H, W)")
+        rv = None if r_vec is None else np.ascontiguousarray(r_vec, dtype=np.float64)
+        check(lib().imexlb_batch_init_fields_f32(self._h, C.c_void_p(fields.data_ptr()), fields.shape[2],
+                                                 fields.shape[3], p_scale, u_scale,
+                                                 None if rv is None else rv.ctypes.data_as(_DP)),
+              "imexlb_batch_init_fields_f32")
+
+    def body_mask_f32(self, out):
+        """Body masks of every case into channel 0 of the device FP32 tensor
+        out (ncase, C, H, W) (1 = body node)."""
+        if out.dim() != 4 or out.shape[0] != self.ncase or not out.is_contiguous():
+            raise ValueError("out: contiguous (ncase, C, H, W)")
+        check(lib().imexlb_batch_body_mask_f32(self._h, C.c_void_p(out.data_ptr()), out.shape[2], out.shape[3],
+                                               out[0].numel()), "imexlb_batch_body_mask_f32")
+
     def advance(self, n):
         check(lib().imexlb_batch_advance(self._h, int(n)), "imexlb_batch_advance")
 

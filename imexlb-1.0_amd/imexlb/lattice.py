@@ -104,6 +104,25 @@ class Lattice:
         u = np.ascontiguousarray(u, dtype=np.float64)
         check(lib().imexlb_upload_fields(self._h, _dptr(p), _dptr(u), layout), "imexlb_upload_fields")
 
+    def init_fields_f32(self, fields, p_scale, u_scale, r_vec=None):
+        """p, u from a device FP32 stack (3, H, W) (a torch tensor on this
+        lattice's GPU): p = p_scale f0, u_x = u_scale f1 (times InitUP's
+        perturbation with r_vec), u_y = u_scale f2; 0 on body nodes."""
+        if fields.dim() != 3 or fields.shape[0] != 3 or not fields.is_contiguous():
+            raise ValueError("fields: contiguous (3, H, W)")
+        rv = None if r_vec is None else np.ascontiguousarray(r_vec, dtype=np.float64)
+        check(lib().imexlb_init_fields_f32(self._h, C.c_void_p(fields.data_ptr()), fields.shape[1], fields.shape[2],
+                                           p_scale, u_scale, None if rv is None else _dptr(rv)),
+              "imexlb_init_fields_f32")
+
+    def body_mask_f32(self, out):
+        """Body mask (1 = body node) of node rows < H, columns < W into the
+        device FP32 tensor out (H, W)."""
+        if out.dim() != 2 or not out.is_contiguous():
+            raise ValueError("out: contiguous (H, W)")
+        check(lib().imexlb_body_mask_f32(self._h, C.c_void_p(out.data_ptr()), out.shape[0], out.shape[1]),
+              "imexlb_body_mask_f32")
+
     def init_pdf(self):
         check(lib().imexlb_init_pdf(self._h), "imexlb_init_pdf")
 

@@ -11,8 +11,9 @@
    that already sheds at the developed amplitude.
 3. Network: a 4-level convolutional encoder-decoder (3x3 convolutions with
    ReLU, 2x2 average pooling, nearest upsampling, skip connections), FP32,
-   trained with Adam by the HIP kernels of include/imexlb_nn.h.  Input:
-   body mask, normalised log Re, and the x and y coordinates.
+   trained with Adam by the HIP kernels of include/imexlb_nn.h and run for
+   the held-out cases in BF16 on the matrix cores (UNetBF16).  Input: body
+   mask, normalised log Re, and the x and y coordinates.
 4. Evaluation on held-out cases: the solver started cold (InitUP) and
    warm (the predicted field through imexlb_batch_init_fields), and warm from
    the training set's mean field (the method without the network); for each
@@ -222,6 +223,83 @@ class UNet:
         return int(sum(w.numel() + b.numel() for w, b in zip(self.w, self.b)))
 
 
+class UNetBF16:
+    """Inference with the weights of a UNet in BF16 on the matrix cores
+    (imexlb_nn_conv3x3_bf16): activations NHWC bf16, every encoder output
+    written straight into its slice of the decoder's concatenated input.
+    Channel counts must be multiples of 16; the input is padded to 16
+    channels.  forward() returns (N, cout, H, W) FP32."""
+
+    def __init__(self, net, N):
+        import torch
+        self.t = torch
+        self.net, self.N = net, N
+        self.chans, self.cout, self.H, self.W = net.chans, net.cout, net.H, net.W
+        if any(c % 16 for c in self.chans):
+            raise ValueError("BF16 inference needs channel counts that are multiples of 16")
+        L, ch = len(self.chans), self.chans
+        self.cinp = 16 * ((net.cin + 15) // 16)
+        bf, dev = torch.bfloat16, "cuda"
+        self.cip = [self.cinp] + [ci for co, ci in net.shapes[1:]]
+        self.cop = [32 * ((co + 31) // 32) for co, ci in net.shapes]
+        self.wp = [torch.empty(self.cip[k] * 9 * self.cop[k], dtype=bf, device=dev) for k in range(len(net.shapes))]
+        self.pack_weights()
+        hw = [(self.H >> l, self.W >> l) for l in range(L)]
+        self.hw = hw
+        e = lambda c, l: torch.empty((N, *hw[l], c), dtype=bf, device=dev)
+        self.x = e(self.cinp, 0)
+        self.t1 = [e(ch[l], l) for l in range(L)]
+        self.cat = [e(ch[l + 1] + ch[l], l) for l in range(L - 1)]
+        self.bottom = e(ch[L - 1], L - 1)
+        self.pooled = [None] + [e(ch[l - 1], l) for l in range(1, L)]
+        self.dec = [e(ch[l], l) for l in range(L - 1)]
+        self.out = torch.empty((N, self.cout, self.H, self.W), dtype=torch.float32, device=dev)
+
+    def pack_weights(self):
+        """(Re)pack the network's current FP32 weights."""
+        for k, (co, ci) in enumerate(self.net.shapes):
+            _ok(lib().imexlb_nn_pack_weights_bf16(_ptr(self.net.w[k]), _ptr(self.wp[k]), ci, co, self.cip[k],
+                                                  self.cop[k], None), "pack weights")
+
+    def _conv(self, k, x, out, level, ostride, ooff, relu=True, f32=False):
+        co, ci = self.net.shapes[k]
+        h, w = self.hw[level]
+        _ok(lib().imexlb_nn_conv3x3_bf16(_ptr(x), _ptr(self.wp[k]), _ptr(self.net.b[k]), _ptr(out), self.N,
+                                         self.cip[k], co, self.cop[k], h, w, ostride, ooff, 1 if relu else 0,
+                                         1 if f32 else 0, None), "conv3x3_bf16")
+
+    def forward(self, x):
+        """x: (N, cin, H, W) FP32 device tensor."""
+        L, ch, N = len(self.chans), self.chans, self.N
+        _ok(lib().imexlb_nn_pack_input_bf16(_ptr(x), _ptr(self.x), N, self.net.cin, self.cinp, self.H, self.W, None),
+            "pack input")
+        k = 0
+        for l in range(L):
+            src = self.x
+            if l > 0:
+                hh, ww = self.hw[l]
+                _ok(lib().imexlb_nn_pool2_bf16(_ptr(self.cat[l - 1]), _ptr(self.pooled[l]), N, ch[l - 1], hh, ww,
+                                               ch[l] + ch[l - 1], ch[l], None), "pool2_bf16")
+                src = self.pooled[l]
+            self._conv(k, src, self.t1[l], l, ch[l], 0)
+            if l < L - 1:
+                self._conv(k + 1, self.t1[l], self.cat[l], l, ch[l + 1] + ch[l], ch[l + 1])
+            else:
+                self._conv(k + 1, self.t1[l], self.bottom, l, ch[l], 0)
+            k += 2
+        d = self.bottom
+        for l in range(L - 2, -1, -1):
+            hh, ww = self.hw[l + 1]
+            _ok(lib().imexlb_nn_up2_bf16(_ptr(d), _ptr(self.cat[l]), N, ch[l + 1], hh, ww, ch[l + 1] + ch[l], 0,
+                                         None), "up2_bf16")
+            self._conv(k, self.cat[l], self.t1[l], l, ch[l], 0)
+            self._conv(k + 1, self.t1[l], self.dec[l], l, ch[l], 0)
+            d = self.dec[l]
+            k += 2
+        self._conv(k, d, self.out, 0, 0, 0, relu=False, f32=True)
+        return self.out
+
+
 # --------------------------------------------------------------------- cases
 def make_cases(n, seed):
     """Seeded cases: list of (Deck, aspect, angle_deg)."""
@@ -260,17 +338,45 @@ def solid_masks(cases):
     return out
 
 
+def norm_log_re(cases):
+    """log Re of every case mapped to [-1, 1] over RE_RANGE (float32)"""
+    lre = np.log([c[0].Re for c in cases])
+    mid, half = np.log(RE_RANGE).mean(), np.ptp(np.log(RE_RANGE)) / 2
+    return ((lre - mid) / half).astype(np.float32)
+
+
+def coords():
+    """x and y input channels (2, NY, NX), each 0..1 across the grid"""
+    c = np.zeros((2, NY, NX), np.float32)
+    c[0] = np.linspace(0.0, 1.0, NX, dtype=np.float32)[None, :]
+    c[1] = np.linspace(0.0, 1.0, NY, dtype=np.float32)[:, None]
+    return c
+
+
 def inputs(cases, masks):
     """Network input (M, 4, NY, NX): mask, normalised log Re, x, y."""
     M = len(cases)
     x = np.zeros((M, 4, NY, NX), np.float32)
-    lre = np.log([c[0].Re for c in cases])
-    mid, half = np.log(RE_RANGE).mean(), np.ptp(np.log(RE_RANGE)) / 2
     x[:, 0] = masks
-    x[:, 1] = ((lre - mid) / half)[:, None, None]
-    x[:, 2] = np.linspace(0.0, 1.0, NX, dtype=np.float32)[None, None, :]
-    x[:, 3] = np.linspace(0.0, 1.0, NY, dtype=np.float32)[None, :, None]
+    x[:, 1] = norm_log_re(cases)[:, None, None]
+    x[:, 2:4] = coords()[None]
     return x
+
+
+def fill_inputs(src, cases, out, coords_dev):
+    """The network input of `cases` built on the GPU in out (M, 4, NY, NX):
+    the body masks from the solver's own geometry (src: a Batch of these
+    cases or, for one case, a Lattice), the normalised log Re and the
+    coordinates (coords_dev: coords() on the device)."""
+    import torch
+    if hasattr(src, "ncase"):
+        src.body_mask_f32(out)
+    else:
+        src.body_mask_f32(out[0, 0])
+    lre = torch.tensor(norm_log_re(cases), device=out.device)
+    out[:, 1].copy_(lre[:, None, None].expand(len(cases), NY, NX))
+    out[:, 2:4].copy_(coords_dev[None].expand(len(cases), 2, NY, NX))
+    src.synchronize()  # the solver's stream wrote the masks
 
 
 def developed_fields(cases, steps, r_vec, precision, probe_every=20, max_probe_steps=6000, log=None):
@@ -344,15 +450,37 @@ def convergence_step(t, drag, ref, window, tol=0.01):
     return int(tt[bad[-1] + 1])
 
 
-def run_history(cases, steps, every, precision, r_vec=None, fields=None):
-    """Batched run from InitUP (r_vec) or from given fields; drag/lift every
-    `every` steps.  Returns (t, F[steps/every, M, 2], seconds)."""
+def warm_start(b, cases, engine, coords_dev, r_vec):
+    """Start the batch `b` of `cases` from the network's prediction, all on
+    the GPU: inputs from the solver's geometry, BF16 inference in batches
+    of engine.N, then imexlb_batch_init_fields_f32.  Returns the
+    predictions (M, 3, NY, NX) as a device tensor."""
+    import torch
+    M, NB = len(cases), engine.N
+    xin = torch.zeros((-(-M // NB) * NB, 4, NY, NX), device="cuda")
+    fill_inputs(b, cases, xin[:M], coords_dev)
+    preds = torch.empty((M, 3, NY, NX), device="cuda")
+    for s in range(0, M, NB):
+        out = engine.forward(xin[s:s + NB])
+        k = min(NB, M - s)
+        preds[s:s + k].copy_(out[:k])
+    torch.cuda.synchronize()  # the batch's stream reads the predictions
+    b.init_fields_f32(preds, UU ** 2, UU, r_vec)
+    return preds
+
+
+def run_history(cases, steps, every, precision, start):
+    """Batched run of `cases` begun by start(batch) (timed separately);
+    drag/lift every `every` steps.  Returns (t, F[steps/every, M, 2],
+    seconds stepping, seconds starting, what start returned)."""
+    import torch
     b = _batch(cases, precision)
-    if fields is None:
-        b.init(r_vec)
-    else:
-        b.init_fields([f[0] for f in fields], [f[1] for f in fields])
     b.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    got = start(b)
+    b.synchronize()
+    t_start = time.perf_counter() - t0
     t0 = time.perf_counter()
     hist, ts = [], []
     for s in range(every, steps + 1, every):
@@ -361,7 +489,7 @@ def run_history(cases, steps, every, precision, r_vec=None, fields=None):
         ts.append(s)
     sec = time.perf_counter() - t0
     b.close()
-    return np.array(ts), np.array(hist), sec
+    return np.array(ts), np.array(hist), sec, t_start, got
 
 
 def main(argv=None):
@@ -421,44 +549,72 @@ def main(argv=None):
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
 
-    # predictions for the held-out cases (a network of the same weights with
-    # a larger batch for inference)
+    # FP32 and BF16 inference of the held-out cases from the same inputs
+    # (time and error only; the warm starts below run BF16 from inputs
+    # built on the GPU)
     te_idx = np.arange(a.train, a.train + a.test)
-    preds = np.zeros((a.test, 3, NY, NX), np.float32)
     NB = min(a.infer_batch, a.test)
-    infer = UNet(4, 3, net.chans, NB, NY, NX, params=net.params_numpy())
-    xi = torch.empty((NB, 4, NY, NX), device="cuda")
-    infer.forward(xi.zero_())  # first launch outside the timing
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for s in range(0, a.test, NB):
-        sel = te_idx[s:s + NB]
-        sel = np.concatenate([sel, np.full(NB - len(sel), sel[-1])])
-        xi.copy_(Xd.index_select(0, torch.tensor(sel, device="cuda")))
-        out = infer.forward(xi)
-        k = min(NB, a.test - s)
-        preds[s:s + k] = out[:k].cpu().numpy()
-    torch.cuda.synchronize()
-    t_infer = time.perf_counter() - t0
-    # one case alone: single-lattice step time and a batch-1 inference
-    one = UNet(4, 3, net.chans, 1, NY, NX, params=net.params_numpy())
-    x1 = torch.empty((1, 4, NY, NX), device="cuda")
-    x1.copy_(Xd[te_idx[0]:te_idx[0] + 1])
-    one.forward(x1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(5):
-        one.forward(x1)
-    torch.cuda.synchronize()
-    t_infer_one = (time.perf_counter() - t0) / 5
+    coords_dev = torch.tensor(coords(), device="cuda")
+
+    def predict(engine):
+        dev = torch.empty((a.test, 3, NY, NX), device="cuda")
+        xi = torch.empty((NB, 4, NY, NX), device="cuda")
+        engine.forward(xi.zero_())  # first launch outside the timing
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in range(0, a.test, NB):
+            sel = te_idx[s:s + NB]
+            sel = np.concatenate([sel, np.full(NB - len(sel), sel[-1])])
+            xi.copy_(Xd.index_select(0, torch.tensor(sel, device="cuda")))
+            out = engine.forward(xi)
+            k = min(NB, a.test - s)
+            dev[s:s + k].copy_(out[:k])
+        torch.cuda.synchronize()
+        return dev.cpu().numpy(), time.perf_counter() - t0
+
+    preds32, t_infer32 = predict(UNet(4, 3, net.chans, NB, NY, NX, params=net.params_numpy()))
+    engine = UNetBF16(net, NB)
+    _, t_infer = predict(engine)
+
+    # one case alone: batch-1 inference, cold and warm starts (the warm one
+    # builds the input from the lattice, infers and sets the fields), and
+    # the single-lattice step time
+    def time_one(eng):
+        x1 = torch.empty((1, 4, NY, NX), device="cuda")
+        x1.copy_(Xd[te_idx[0]:te_idx[0] + 1])
+        eng.forward(x1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            eng.forward(x1)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / 5
+
+    t_infer_one32 = time_one(UNet(4, 3, net.chans, 1, NY, NX, params=net.params_numpy()))
+    one = UNetBF16(net, 1)
+    t_infer_one = time_one(one)
     from .lattice import Lattice
     dk, asp, ang = test[0]
     dd = derive(dk)
     lat = Lattice(2, NX, NY, 0, device=0, precision=prec)
     lat.set_params(dk.ome, dk.rho0, dd.uu)
     lat.set_geometry(dd.pos, dd.radius, asp, ang)
-    lat.init_up(r_vec)
-    lat.init_pdf()
+    x1 = torch.empty((1, 4, NY, NX), device="cuda")
+    starts = {}
+    for kind in ("cold", "warm", "cold", "warm"):  # the second of each is timed
+        lat.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if kind == "cold":
+            lat.init_up(r_vec)
+        else:
+            fill_inputs(lat, test[:1], x1, coords_dev)
+            pred = one.forward(x1)
+            torch.cuda.synchronize()
+            lat.init_fields_f32(pred[0], UU ** 2, UU, r_vec)
+        lat.init_pdf()
+        lat.synchronize()
+        starts[kind] = time.perf_counter() - t0
     lat.advance(64)
     lat.synchronize()
     t0 = time.perf_counter()
@@ -466,21 +622,30 @@ def main(argv=None):
     lat.synchronize()
     t_step_one = (time.perf_counter() - t0) / 4096
     lat.close()
+
+    # cold, warm (network), warm (mean field)
+    mean_field = Y[tr_idx].mean(axis=0)
+    mean_dev = torch.tensor(np.repeat(mean_field[None], a.test, axis=0), device="cuda")
+    every = 100
+    tc, Fc, sc, ic, _ = run_history(test, a.eval_steps, every, prec, lambda b: b.init(r_vec))
+    tw, Fw, sw, iw, preds_dev = run_history(test, a.eval_steps, every, prec,
+                                            lambda b: warm_start(b, test, engine, coords_dev, r_vec))
+    tm, Fm, sm, im, _ = run_history(test, a.eval_steps, every, prec,
+                                    lambda b: b.init_fields_f32(mean_dev, UU ** 2, UU, r_vec))
+    preds = preds_dev.cpu().numpy()
+    inputs_dev = torch.zeros((a.test, 4, NY, NX), device="cuda")
+    bchk = _batch(test, prec)
+    fill_inputs(bchk, test, inputs_dev, coords_dev)
+    bchk.close()
+    inputs_match = bool(np.array_equal(inputs_dev.cpu().numpy(), X[te_idx]))
     held = Y[te_idx]
     ok = finite[te_idx]
     err = float(np.sqrt(np.mean((preds[ok] - held[ok]) ** 2)) / np.sqrt(np.mean(held[ok] ** 2)))
-    mean_field = Y[tr_idx].mean(axis=0)
+    err32 = float(np.sqrt(np.mean((preds32[ok] - held[ok]) ** 2)) / np.sqrt(np.mean(held[ok] ** 2)))
     err_mean = float(np.sqrt(np.mean((mean_field[None] - held[ok]) ** 2)) / np.sqrt(np.mean(held[ok] ** 2)))
-    log(f"[surrogate] held-out rel. RMS error: network {err:.3f}, mean field {err_mean:.3f}")
+    log(f"[surrogate] held-out rel. RMS error: network {err:.3f} (BF16), {err32:.3f} (FP32), "
+        f"mean field {err_mean:.3f}")
 
-    # cold, warm (network), warm (mean field)
-    solid = masks[te_idx][:, None]
-    warm = [to_lattice_fields(np.where(solid[i] > 0, 0.0, preds[i]), r_vec) for i in range(a.test)]
-    meanw = [to_lattice_fields(np.where(solid[i] > 0, 0.0, mean_field), r_vec) for i in range(a.test)]
-    every = 100
-    tc, Fc, sc = run_history(test, a.eval_steps, every, prec, r_vec=r_vec)
-    tw, Fw, sw = run_history(test, a.eval_steps, every, prec, fields=warm)
-    tm, Fm, sm = run_history(test, a.eval_steps, every, prec, fields=meanw)
     window = 2560  # about one shedding period at St 0.2
     tail = max(1, len(tc) // 4)
     res = []
@@ -499,17 +664,20 @@ def main(argv=None):
     valid = [r for r in res if r["cold"] and r["warm"] and r["drag_match"] and r["lift_match"]]
     ratio = [r["warm"] / r["cold"] for r in valid]
     # wall time of one case: its steps at the batched rate per case plus its
-    # share of the batched inference; and run alone: single-lattice steps
-    # plus a batch-1 inference
+    # share of the batched inference and start; and run alone:
+    # single-lattice steps plus a batch-1 inference and its start
     step_case = sc / a.eval_steps / a.test
-    wall = [(r["warm"] * step_case + t_infer / a.test) / (r["cold"] * step_case) for r in valid]
-    wall_one = [(r["warm"] * t_step_one + t_infer_one) / (r["cold"] * t_step_one) for r in valid]
+    wall = [(r["warm"] * step_case + iw / a.test) / (r["cold"] * step_case + ic / a.test) for r in valid]
+    wall_one = [(r["warm"] * t_step_one + starts["warm"]) / (r["cold"] * t_step_one + starts["cold"])
+                for r in valid]
     ratio_m = [r["mean"] / r["cold"] for r in valid if r["mean"]]
-    out = {"train_cases": int(len(tr_idx)), "test_cases": a.test, "params": net.n_params(), "chans": a.chans,
+    out = {"train_cases": int(len(tr_idx)), "inputs_built_on_gpu_match": inputs_match, "test_cases": a.test, "params": net.n_params(), "chans": a.chans,
            "infer_batch": NB,
            "epochs": a.epochs, "data_s": round(t_data, 1), "train_s": round(t_train, 1),
-           "infer_s_per_case": round(t_infer / a.test, 5), "loss_curve": [round(c, 5) for c in curve],
-           "heldout_rel_rms": round(err, 4), "mean_field_rel_rms": round(err_mean, 4),
+           "infer_s_per_case": round(t_infer / a.test, 6), "infer_fp32_s_per_case": round(t_infer32 / a.test, 6),
+           "loss_curve": [round(c, 5) for c in curve],
+           "heldout_rel_rms": round(err, 4), "heldout_rel_rms_fp32": round(err32, 4),
+           "mean_field_rel_rms": round(err_mean, 4),
            "cases_scored": len(valid), "cases_evaluated": len(res),
            "drag_match": int(sum(r["drag_match"] for r in res)), "lift_match": int(sum(r["lift_match"] for r in res)),
            "median_steps_cold": float(np.median([r["cold"] for r in valid])) if valid else None,
@@ -520,8 +688,11 @@ def main(argv=None):
            "median_wall_ratio_warm_over_cold": float(np.median(wall)) if wall else None,
            "frac_wall_within_one_third": float(np.mean(np.array(wall) <= 1.0 / 3.0)) if wall else None,
            "median_wall_ratio_single_case": float(np.median(wall_one)) if wall_one else None,
-           "single_case": {"step_ms": round(t_step_one * 1e3, 5), "infer_ms": round(t_infer_one * 1e3, 3)},
-           "batch_seconds": {"cold": round(sc, 2), "warm": round(sw, 2), "mean_field": round(sm, 2)}}
+           "single_case": {"step_ms": round(t_step_one * 1e3, 5), "infer_ms": round(t_infer_one * 1e3, 3),
+                           "infer_fp32_ms": round(t_infer_one32 * 1e3, 3)},
+           "batch_seconds": {"cold": round(sc, 2), "warm": round(sw, 2), "mean_field": round(sm, 2)},
+           "start_seconds": {"batch_cold": round(ic, 4), "batch_warm": round(iw, 4),
+                             "single_cold": round(starts["cold"], 5), "single_warm": round(starts["warm"], 5)}}
     print(json.dumps(out), flush=True)
 
 

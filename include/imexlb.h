@@ -122,6 +122,23 @@ int imexlb_init_up(imexlb_ctx *ctx, const double *r_vec);
 /* Warm start: replace p and u (layout IMEXLB_LAYOUT_*), e.g. with a
  * predicted field; follow with imexlb_init_pdf (lbm.f90:24-48). */
 int imexlb_upload_fields(imexlb_ctx *ctx, const double *p, const double *u, int layout);
+/* Warm start of a 2D lattice from device memory (no counterpart in the
+ * reference; for callers whose predicted fields are already on the GPU):
+ * fields is a device FP32 stack [3][H][W] of p, u_x, u_y at node rows
+ * y < H and columns x < W (nodes beyond take the last row and column);
+ * p = p_scale f0, u_x = u_scale f1 r, u_y = u_scale f2, 0 on body nodes,
+ * with r InitUP's perturbation factor 1 + 0.4 (r_vec[id mod 100] - 0.5)
+ * (initialization.f90:273) when r_vec is given (NULL: r = 1).  Follow
+ * with imexlb_init_pdf.  The stack must be complete: the call does not wait
+ * for work on other streams. */
+int imexlb_init_fields_f32(imexlb_ctx *ctx, const float *fields, int H, int W, double p_scale, double u_scale,
+                           const double *r_vec);
+/* Body mask of a 2D lattice into device memory (no counterpart in the
+ * reference; the input of a field predictor): out[y W + x] = 1 on body
+ * nodes, 0 on fluid nodes, for node rows y < H <= ny+1 and columns
+ * x < W <= nx+1.  Asynchronous on the context's stream (imexlb_synchronize
+ * before other streams read it). */
+int imexlb_body_mask_f32(imexlb_ctx *ctx, float *out, int H, int W);
 /* InitPDF (lbm.f90:24-48): f = feq(p, u) on fluid nodes. */
 int imexlb_init_pdf(imexlb_ctx *ctx);
 
@@ -204,6 +221,14 @@ int imexlb_batch_init(imexlb_batch *b, const double *r_vec);
  * interior layout of imexlb_batch_download_fields ((nx+1)(ny+1) values of
  * p; u[d][y][x]). */
 int imexlb_batch_init_fields(imexlb_batch *b, const double *p, const double *u);
+/* Warm start of every case from device memory: fields holds ncase FP32
+ * stacks [3][H][W], one per case in case order, each applied as
+ * imexlb_init_fields_f32 does; then InitPDF + the first Collision. */
+int imexlb_batch_init_fields_f32(imexlb_batch *b, const float *fields, int H, int W, double p_scale,
+                                 double u_scale, const double *r_vec);
+/* Body masks of every case as imexlb_body_mask_f32 writes them, case i at
+ * out + i stride (stride >= H W); asynchronous on the batch's stream. */
+int imexlb_batch_body_mask_f32(imexlb_batch *b, float *out, int H, int W, int64_t stride);
 /* nsteps complete iterations of every case. */
 int imexlb_batch_advance(imexlb_batch *b, int64_t nsteps);
 /* Step-kernel shape: threads per block (64, 128, 256, 512), nodes per

@@ -45,6 +45,32 @@ int imexlb_nn_mse(const float *pred, const float *target, float *grad, int64_t n
 int imexlb_nn_adam(float *w, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2, float eps,
                    int step, void *stream);
 
+/* BF16 inference.  Activations are NHWC bf16 (uint16_t bit patterns),
+ * weights packed by imexlb_nn_pack_weights_bf16; products accumulate in
+ * FP32 on the matrix cores; values are rounded to bf16 to nearest even.
+ * Buffers given as uint16_t pointers must be 16-byte aligned. */
+/* x [N][C][H][W] FP32 -> y [N][H][W][CP] bf16 (channels C..CP-1 zero) */
+int imexlb_nn_pack_input_bf16(const float *x, uint16_t *y, int N, int C, int CP, int H, int W, void *stream);
+/* w [CO][CI][3][3] FP32 -> wp [CIP/8][9][COP][8] bf16, zero-padded;
+ * CIP a multiple of 8, COP >= CO */
+int imexlb_nn_pack_weights_bf16(const float *w, uint16_t *wp, int CI, int CO, int CIP, int COP, void *stream);
+/* 3x3 convolution, padding 1, of in [N][H][W][CIP] (CIP a multiple of 16)
+ * with packed wp (COP a multiple of 32): output channel co < CO of pixel
+ * (n, y, x) = act(bias[co] + sum), written as bf16 to
+ * out[((n H + y) W + x) ostride + ooff + co] (ooff + CO <= ostride), or,
+ * with f32_nchw = 1, as FP32 to out[((n CO + co) H + y) W + x].  bias may
+ * be NULL; relu: 1 applies max(0, .); N <= 65535. */
+int imexlb_nn_conv3x3_bf16(const uint16_t *in, const uint16_t *wp, const float *bias, void *out, int N, int CIP,
+                           int CO, int COP, int H, int W, int ostride, int ooff, int relu, int f32_nchw, void *stream);
+/* 2x2 average pool of channels [coff, coff+C) of in [N][2h][2w][cstride]
+ * into out [N][h][w][C] (C, cstride, coff multiples of 8) */
+int imexlb_nn_pool2_bf16(const uint16_t *in, uint16_t *out, int N, int C, int h, int w, int cstride, int coff,
+                         void *stream);
+/* nearest 2x upsample of in [N][h][w][C] into channels [coff, coff+C) of
+ * out [N][2h][2w][Ctot] (C, Ctot, coff multiples of 8) */
+int imexlb_nn_up2_bf16(const uint16_t *in, uint16_t *out, int N, int C, int h, int w, int Ctot, int coff,
+                       void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,7 +1,9 @@
 """NumPy restatement of the warm-start network (test infrastructure): the
 same encoder-decoder as imexlb.surrogate.UNet, forward and backward, in
-float64, used to check the HIP kernels' outputs and gradients.  Its own
-gradients are pinned by finite differences (tests/test_nn_numpy.py)."""
+float64, used to check the HIP kernels' outputs and gradients, and its BF16
+inference form (imexlb.surrogate.UNetBF16).  Its own gradients are pinned
+by finite differences and its bf16 rounding by torch's conversion
+(tests/test_nn_numpy.py)."""
 import numpy as np
 
 
@@ -115,3 +117,50 @@ def backward(dout, params, chans, cache):
         if l > 0:
             g = pool2_backward(g) + denc[l - 1]
     return grads
+
+
+# ------------------------------------------------------------- BF16 inference
+def bf16(x):
+    """Round to bfloat16 through float32, to nearest even (finite values);
+    returns float64."""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return (u & 0xFFFFFFFF).astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def pool2_bf16(x):
+    """2x2 average of bf16 values in float32, ((a + b) + (c + d)) / 4, rounded."""
+    f = np.asarray(x, np.float32)
+    s = (f[:, :, 0::2, 0::2] + f[:, :, 0::2, 1::2]) + (f[:, :, 1::2, 0::2] + f[:, :, 1::2, 1::2])
+    return bf16(np.float32(0.25) * s)
+
+
+def pack_weights_bf16(w, cip, cop):
+    """[CO][CI][3][3] -> [CIP/8][9][COP][8] of bf16 values (zero-padded)."""
+    co, ci = w.shape[:2]
+    full = np.zeros((cop, cip, 9))
+    full[:co, :ci] = bf16(w).reshape(co, ci, 9)
+    return full.reshape(cop, cip // 8, 8, 9).transpose(1, 3, 0, 2).copy()
+
+
+def forward_bf16(x, params, chans):
+    """The network as its BF16 inference computes it: input, weights and
+    every hidden activation rounded to bf16; sums, bias and ReLU exact
+    (float64 here, FP32 on the GPU); the output not rounded."""
+    it = iter(params)
+
+    def conv(inp, relu=True, last=False):
+        w, b = next(it)
+        o = conv3x3(inp, bf16(w), np.asarray(b, np.float32).astype(np.float64), relu)
+        return o if last else bf16(o)
+
+    enc, h = [], bf16(x)
+    for l in range(len(chans)):
+        if l > 0:
+            h = pool2_bf16(h)
+        h = conv(conv(h))
+        enc.append(h)
+    d = enc[-1]
+    for l in range(len(chans) - 2, -1, -1):
+        d = conv(conv(np.concatenate([up2(d), enc[l]], axis=1)))
+    return conv(d, relu=False, last=True)

@@ -143,3 +143,131 @@ def test_batch_warm_start_matches_single_lattices(r_vec):
         assert np.array_equal(lat.forces(), Fb[i]), i
         lat.close()
     b.close()
+
+
+def _fluid(dk, shape=(1.0, 0.0)):
+    """Fluid flags (ny+1, nx+1) of a deck's nodes from the geometry table."""
+    d = I.derive(dk)
+    nx, ny = dk.n[0], dk.n[1]
+    lat = I.Lattice(2, nx, ny, 0, device=-1)
+    lat.set_params(dk.ome, dk.rho0, d.uu)
+    lat.set_geometry(d.pos, d.radius, *shape)
+    geo = lat.geometry_geo().reshape(ny + 7, nx + 7)[3:ny + 4, 3:nx + 4]
+    lat.close()
+    return (geo != 0).astype(np.float64)
+
+
+def _stack_fields(dk, f, ps, us, r_vec, shape=(1.0, 0.0)):
+    """p, u (interior layout) that an FP32 stack f (3, H, W) gives a deck's
+    nodes: node (x, y) reads f[:, min(y, H-1), min(x, W-1)]; p = ps f0,
+    u_x = us f1 r, u_y = us f2, r = 1 + ((r_vec[id mod 100] - 0.5) 2) 0.2
+    with the reference id of node (x, y) (initialization.f90:273), r = 1
+    without r_vec; zero on body nodes."""
+    nx, ny = dk.n[0], dk.n[1]
+    H, W = f.shape[1:]
+    y, x = np.mgrid[0:ny + 1, 0:nx + 1]
+    g = f[:, np.minimum(y, H - 1), np.minimum(x, W - 1)].astype(np.float64)
+    r = 1.0 if r_vec is None else 1.0 + ((np.asarray(r_vec)[((x + 3) + (nx + 7) * (y + 3)) % 100] - 0.5) * 2.0) * 0.2
+    fl = _fluid(dk, shape)
+    return (ps * g[0] * fl).ravel(), np.concatenate([(us * g[1] * r * fl).ravel(), (us * g[2] * fl).ravel()])
+
+
+@pytest.mark.gpu
+def test_init_fields_f32_single_lattice(r_vec):
+    """imexlb_init_fields_f32 sets p, u as the stack formula states (bit
+    for bit), and a stack of InitUP's normalised inflow profile reproduces
+    InitUP (up to the profile's FP32 rounding), which pins the perturbation's
+    node indexing."""
+    import torch
+    dk = _small_cases()[1]
+    nx, ny = dk.n[0], dk.n[1]
+    d = I.derive(dk)
+    rng = np.random.default_rng(31)
+    f = rng.normal(0.0, 1.0, (3, ny, nx - 5)).astype(np.float32)  # last row and 6 columns repeat
+    fd = torch.tensor(f, device="cuda")
+    lat, _ = I.setup(dk, r_vec, device=0)
+    for rv in (r_vec, None):
+        lat.init_fields_f32(fd, 2.5e-3, 0.05, rv)
+        p, u = lat.download_fields()
+        pe, ue = _stack_fields(dk, f, 2.5e-3, 0.05, rv)
+        assert np.array_equal(p, pe) and np.array_equal(u, ue)
+    y = np.arange(ny + 1, dtype=np.float64)
+    prof = np.zeros((3, ny + 1, nx + 1), np.float32)
+    prof[1] = (4.0 * y * (ny - y) / float(ny * ny))[:, None]
+    lat.init_fields_f32(torch.tensor(prof, device="cuda"), 1.0, d.uu, r_vec)
+    p, u = lat.download_fields()
+    lat.init_up(r_vec)
+    p0, u0 = lat.download_fields()
+    assert not p.any() and not p0.any()
+    assert np.allclose(u, u0, rtol=1e-6, atol=0.0)
+    lat.init_pdf()
+    lat.advance(5)
+    assert np.isfinite(lat.download_fields()[1]).all()
+    lat.close()
+
+
+@pytest.mark.gpu
+def test_batch_init_fields_f32_matches_host_fields(r_vec):
+    """A batch warm-started from a device FP32 stack
+    (imexlb_batch_init_fields_f32) equals the same batch warm-started from
+    the host arrays the stack formula gives (imexlb_batch_init_fields), bit
+    for bit, at the start and after 17 steps."""
+    import torch
+    decks = _small_cases()[:3]
+    shapes = [(1.0, 0.0), (1.6, 35.0), (0.7, 80.0)]
+    H = min(dk.n[1] for dk in decks)
+    W = min(dk.n[0] for dk in decks)
+    rng = np.random.default_rng(32)
+    f = np.zeros((3, 3, H, W), np.float32)
+    f[:, 0] = rng.normal(0.0, 0.2, (3, H, W))
+    f[:, 1] = 1.0 + rng.normal(0.0, 0.1, (3, H, W))
+    f[:, 2] = rng.normal(0.0, 0.1, (3, H, W))
+    host = [_stack_fields(dk, f[i], 2.5e-3, 0.05, r_vec, shapes[i]) for i, dk in enumerate(decks)]
+    a = Batch(decks, precision=I.F64, shapes=shapes)
+    a.init_fields([h[0] for h in host], [h[1] for h in host])
+    b = Batch(decks, precision=I.F64, shapes=shapes)
+    b.init_fields_f32(torch.tensor(f, device="cuda"), 2.5e-3, 0.05, r_vec)
+    for steps in (0, 17):
+        a.advance(steps)
+        b.advance(steps)
+        for i in range(3):
+            pa, ua = a.download_fields(i)
+            pb, ub = b.download_fields(i)
+            assert np.isfinite(pa).all() and np.array_equal(pa, pb) and np.array_equal(ua, ub), (steps, i)
+        if steps:  # forces need a completed step
+            assert np.array_equal(a.forces(), b.forces())
+    a.close()
+    b.close()
+
+
+@pytest.mark.gpu
+def test_body_masks_match_geometry():
+    """imexlb_body_mask_f32 and imexlb_batch_body_mask_f32 write 1 exactly on
+    the geometry's body nodes, leave other channels alone, and reject masks
+    larger than a case."""
+    import torch
+    decks = _small_cases()[:3]
+    shapes = [(1.0, 0.0), (1.6, 35.0), (0.7, 80.0)]
+    H = min(dk.n[1] for dk in decks) + 1
+    W = min(dk.n[0] for dk in decks) + 1
+    b = Batch(decks, precision=I.F64, shapes=shapes)
+    out = torch.full((3, 2, H, W), 7.0, device="cuda")
+    b.body_mask_f32(out)
+    b.synchronize()
+    got = out.cpu().numpy()
+    for i, dk in enumerate(decks):
+        assert np.array_equal(got[i, 0], 1.0 - _fluid(dk, shapes[i])[:H, :W]), i
+        assert (got[i, 1] == 7.0).all()
+    with pytest.raises(I.ImexlbError):
+        b.body_mask_f32(torch.zeros((3, 1, H + 1, W), device="cuda"))
+    b.close()
+    dk = decks[1]
+    d = I.derive(dk)
+    lat = I.Lattice(2, dk.n[0], dk.n[1], 0, device=0)
+    lat.set_params(dk.ome, dk.rho0, d.uu)
+    lat.set_geometry(d.pos, d.radius, *shapes[1])
+    m = torch.zeros((dk.n[1] + 1, dk.n[0] + 1), device="cuda")
+    lat.body_mask_f32(m)
+    lat.synchronize()
+    assert np.array_equal(m.cpu().numpy(), 1.0 - _fluid(dk, shapes[1]))
+    lat.close()

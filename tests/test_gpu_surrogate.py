@@ -1,7 +1,9 @@
 """The warm-start network's HIP kernels (include/imexlb_nn.h) against the
 NumPy restatement (tests/nn_numpy.py): forward output and every weight and
-bias gradient of a small encoder-decoder, and a short training run whose
-loss falls."""
+bias gradient of a small encoder-decoder, a short training run whose loss
+falls, and the BF16 inference kernels (packing, pooling and upsampling bit
+for bit; the matrix-core convolution to one bf16 rounding of the exact sum;
+the whole BF16 network against its NumPy form)."""
 import numpy as np
 import pytest
 
@@ -55,3 +57,120 @@ def test_training_reduces_the_loss():
         net.backward(net.grad_out)
         net.adam(3e-3)
     assert loss < 0.3 * first, (first, loss)
+
+
+def _bf16_nchw(t):
+    """NHWC bf16 device tensor -> NCHW float64 array"""
+    return t.float().cpu().numpy().astype(np.float64).transpose(0, 3, 1, 2)
+
+
+def _lib():
+    from imexlb._lib import lib
+    return lib()
+
+
+def _p(t):
+    import ctypes
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def test_bf16_pack_pool_upsample_bit_exact():
+    import torch
+    L = _lib()
+    rng = np.random.default_rng(21)
+    N, C, H, W = 2, 5, 12, 20
+    x = rng.normal(0, 3, (N, C, H, W)).astype(np.float32)
+    xd = torch.tensor(x, device="cuda")
+    y = torch.empty((N, H, W, 16), dtype=torch.bfloat16, device="cuda")
+    assert L.imexlb_nn_pack_input_bf16(_p(xd), _p(y), N, C, 16, H, W, None) == 0
+    got = _bf16_nchw(y)
+    assert np.array_equal(got[:, :C], R.bf16(x)) and not got[:, C:].any()
+    w = rng.normal(0, 1, (40, 24, 3, 3)).astype(np.float32)
+    wd = torch.tensor(w, device="cuda")
+    wp = torch.empty(24 * 9 * 64, dtype=torch.bfloat16, device="cuda")
+    assert L.imexlb_nn_pack_weights_bf16(_p(wd), _p(wp), 24, 40, 24, 64, None) == 0
+    assert np.array_equal(wp.float().cpu().numpy(), R.pack_weights_bf16(w, 24, 64).ravel())
+    # pool channels [16, 40) of a 48-channel buffer; upsample into [8, 32) of another
+    src = R.bf16(rng.normal(0, 2, (N, 48, H, W)))
+    sd = torch.tensor(src.transpose(0, 2, 3, 1).astype(np.float32), device="cuda").to(torch.bfloat16).contiguous()
+    pd = torch.empty((N, H // 2, W // 2, 24), dtype=torch.bfloat16, device="cuda")
+    assert L.imexlb_nn_pool2_bf16(_p(sd), _p(pd), N, 24, H // 2, W // 2, 48, 16, None) == 0
+    pooled = R.pool2_bf16(src[:, 16:40])
+    assert np.array_equal(_bf16_nchw(pd), pooled)
+    ud = torch.zeros((N, H, W, 40), dtype=torch.bfloat16, device="cuda")
+    assert L.imexlb_nn_up2_bf16(_p(pd), _p(ud), N, 24, H // 2, W // 2, 40, 8, None) == 0
+    up = _bf16_nchw(ud)
+    assert np.array_equal(up[:, 8:32], R.up2(pooled)) and not up[:, :8].any() and not up[:, 32:].any()
+    # argument checks
+    assert L.imexlb_nn_pool2_bf16(_p(sd), _p(pd), N, 20, H // 2, W // 2, 48, 16, None) == -1
+    assert L.imexlb_nn_up2_bf16(_p(pd), _p(ud), N, 24, H // 2, W // 2, 40, 24, None) == -1
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("cip,co,H,W,relu,f32,ostride,ooff", [
+    (16, 32, 20, 36, 1, 0, 32, 0),      # 16-channel chunks, 32-channel blocks, ragged tiles
+    (32, 64, 8, 16, 0, 0, 96, 24),      # one tile, output into a channel slice
+    (96, 128, 13, 50, 1, 0, 128, 0),    # 64-channel blocks, two of them
+    (48, 3, 9, 17, 0, 1, 0, 0),         # FP32 NCHW output of 3 channels
+    (384, 128, 16, 64, 1, 0, 128, 0),   # the deepest decoder input of the warm-start network
+    (64, 48, 24, 40, 1, 0, 48, 0),      # CO not a multiple of 32
+])
+def test_bf16_conv_matches_exact_sum(cip, co, H, W, relu, f32, ostride, ooff):
+    import torch
+    L = _lib()
+    print(f"[surrogate] bf16 conv CIP={cip} CO={co} {H}x{W}", flush=True)
+    rng = np.random.default_rng(cip * 7 + co)
+    N = 2
+    x = rng.normal(0, 1, (N, cip, H, W)).astype(np.float32)
+    w = rng.normal(0, np.sqrt(2.0 / (9 * cip)), (co, cip, 3, 3)).astype(np.float32)
+    b = rng.normal(0, 0.3, co).astype(np.float32)
+    cop = 32 * ((co + 31) // 32)
+    xd, wd, bd = (torch.tensor(a, device="cuda") for a in (x, w, b))
+    xin = torch.empty((N, H, W, cip), dtype=torch.bfloat16, device="cuda")
+    wp = torch.empty(cip * 9 * cop, dtype=torch.bfloat16, device="cuda")
+    assert L.imexlb_nn_pack_input_bf16(_p(xd), _p(xin), N, cip, cip, H, W, None) == 0
+    assert L.imexlb_nn_pack_weights_bf16(_p(wd), _p(wp), cip, co, cip, cop, None) == 0
+    if f32:
+        out = torch.empty((N, co, H, W), dtype=torch.float32, device="cuda")
+    else:
+        out = torch.zeros((N, H, W, ostride), dtype=torch.bfloat16, device="cuda")
+    assert L.imexlb_nn_conv3x3_bf16(_p(xin), _p(wp), _p(bd), _p(out), N, cip, co, cop, H, W, ostride, ooff, relu,
+                                    f32, None) == 0
+    ref = R.conv3x3(R.bf16(x), R.bf16(w), b.astype(np.float64), bool(relu))
+    if f32:
+        got = out.cpu().numpy().astype(np.float64)
+    else:
+        full = _bf16_nchw(out)
+        got = full[:, ooff:ooff + co]
+        assert not full[:, :ooff].any() and not full[:, ooff + co:].any()
+    bound = 2.0 ** -8 * np.abs(ref) + 1e-4 * np.abs(ref).max()
+    assert (np.abs(got - ref) <= bound).all(), np.abs(got - ref).max()
+
+
+def test_bf16_conv_rejects_bad_arguments():
+    import torch
+    L = _lib()
+    a = torch.zeros(64 * 9 * 64, dtype=torch.bfloat16, device="cuda")
+    o = torch.zeros(64 * 64, dtype=torch.bfloat16, device="cuda")
+    assert L.imexlb_nn_conv3x3_bf16(_p(a), _p(a), None, _p(o), 1, 24, 32, 32, 4, 4, 32, 0, 1, 0, None) == -1
+    assert L.imexlb_nn_conv3x3_bf16(_p(a), _p(a), None, _p(o), 1, 32, 40, 40, 4, 4, 40, 0, 1, 0, None) == -1
+    assert L.imexlb_nn_conv3x3_bf16(_p(a), _p(a), None, _p(o), 1, 32, 32, 32, 4, 4, 40, 16, 1, 0, None) == -1
+    assert L.imexlb_nn_pack_weights_bf16(_p(a), _p(a), 12, 8, 12, 32, None) == -1
+
+
+def test_bf16_network_matches_numpy():
+    import torch
+    from imexlb.surrogate import UNetBF16
+    net, x, chans = _setup(chans=(16, 32, 64), N=2, H=32, W=64)
+    print("[surrogate] bf16 network vs numpy", flush=True)
+    params = net.params_numpy()
+    xd = torch.tensor(x, device="cuda")
+    eng = UNetBF16(net, 2)
+    got = eng.forward(xd).cpu().numpy().astype(np.float64)
+    ref = R.forward_bf16(x.astype(np.float64), params, chans)
+    full, _ = R.forward(x.astype(np.float64), params, chans)
+    rel = lambda a, b: np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2))
+    assert rel(got, ref) < 1e-2, rel(got, ref)
+    assert rel(got, full) < 3e-2, rel(got, full)
+    out32 = net.forward(xd).cpu().numpy()
+    assert rel(got, out32) < 3e-2

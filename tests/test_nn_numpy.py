@@ -55,3 +55,36 @@ def test_gradients_match_finite_differences():
         b[0] = keep
         assert abs((lp - lm) / (2 * eps) - grads[li][1][0]) <= 1e-6 * max(1.0, abs(lp - lm) / (2 * eps))
     assert checked == 8
+
+
+def test_bf16_rounding_matches_torch():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(11)
+    x = np.concatenate([rng.normal(0, 1, 4000), rng.normal(0, 1e-30, 500), rng.normal(0, 1e30, 500),
+                        [0.0, -0.0, 1.0, 1.00390625, 1.005859375, 1.001953125, 3.0e38]]).astype(np.float32)
+    ref = torch.tensor(x).to(torch.bfloat16).to(torch.float64).numpy()
+    got = R.bf16(x)
+    assert np.array_equal(got, ref)
+    # ties go to the even mantissa: 1 + 2^-8 lies halfway between 1 and 1 + 2^-7
+    assert R.bf16(np.float32(1.00390625)) == 1.0
+    assert R.bf16(np.float32(1.01171875)) == 1.015625
+
+
+def test_bf16_pieces():
+    rng = np.random.default_rng(12)
+    x = R.bf16(rng.normal(size=(2, 4, 6, 8)))
+    assert np.array_equal(R.pool2_bf16(R.up2(x)), x)
+    w = rng.normal(size=(5, 12, 3, 3))
+    wp = R.pack_weights_bf16(w, 16, 32).ravel()
+    for co, ci, t in [(0, 0, 0), (4, 11, 8), (2, 7, 4), (3, 8, 5)]:
+        g, j = divmod(ci, 8)
+        assert wp[((g * 9 + t) * 32 + co) * 8 + j] == R.bf16(w[co, ci, t // 3, t % 3])
+    assert wp.size == 16 * 9 * 32 and np.count_nonzero(wp) <= 5 * 12 * 9
+
+
+def test_bf16_forward_is_close_to_float64():
+    params, x, chans = _net(chans=(3, 4))
+    ref, _ = R.forward(x, params, chans)
+    got = R.forward_bf16(x, params, chans)
+    err = np.sqrt(np.mean((got - ref) ** 2)) / np.sqrt(np.mean(ref ** 2))
+    assert 0 < err < 2e-2

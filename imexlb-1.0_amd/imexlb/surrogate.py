@@ -1,6 +1,7 @@
 """Learned warm start for D2Q9 cylinder/ellipse cases.
 
     python -m imexlb.surrogate [--train 2000] [--test 200] [--epochs 20]
+    python -m imexlb.surrogate --inference-only
 
 1. Cases: 512x128 channel, a body of the cylinder's area (radius Ny/10) at
    x = 0.2 Nx, a circle or an ellipse of axis ratio U[0.5, 2] turned by
@@ -492,6 +493,40 @@ def run_history(cases, steps, every, precision, start):
     return np.array(ts), np.array(hist), sec, t_start, got
 
 
+def conv_flops(shapes, H, W, levels):
+    """Multiply-add FLOPs (2 per MAC) of one forward pass of one case."""
+    L = len(levels)
+    return sum(2 * 9 * co * ci * (H >> levels[k]) * (W >> levels[k]) for k, (co, ci) in enumerate(shapes))
+
+
+def bench_inference(chans, sizes=(40, 8, 1), reps=10):
+    """Time FP32 and BF16 inference of the default network (random
+    weights) per batch size; one JSON line."""
+    import torch
+    out = {"grid": [NY, NX], "chans": chans, "runs": []}
+    for N in sizes:
+        net = UNet(4, 3, chans, N, NY, NX, seed=1)
+        eng = UNetBF16(net, N)
+        flops = conv_flops(net.shapes, NY, NX, net.level_of)
+        x = torch.randn((N, 4, NY, NX), device="cuda")
+        row = {"batch": N, "gflop_per_case": round(flops / 1e9, 3)}
+        for name, e, r in (("fp32", net, max(1, reps // 3)), ("bf16", eng, reps)):
+            e.forward(x)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(r):
+                e.forward(x)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / r
+            row[name] = {"ms_per_batch": round(dt * 1e3, 4), "ms_per_case": round(dt / N * 1e3, 4),
+                         "tflops": round(flops * N / dt / 1e12, 1)}
+        a32, b16 = net.forward(x).cpu().numpy(), eng.forward(x).cpu().numpy()
+        row["bf16_rel_rms_vs_fp32"] = float(np.sqrt(np.mean((a32 - b16) ** 2)) / np.sqrt(np.mean(a32 ** 2)))
+        out["runs"].append(row)
+        print(f"[surrogate] inference batch {N}: {row}", file=sys.stderr, flush=True)
+    print(json.dumps(out), flush=True)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=2000)
@@ -504,7 +539,11 @@ def main(argv=None):
     ap.add_argument("--infer-batch", type=int, default=40)
     ap.add_argument("--eval-steps", type=int, default=40000)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--inference-only", action="store_true",
+                    help="time FP32 and BF16 inference of the network (random weights) and stop")
     a = ap.parse_args(argv)
+    if a.inference_only:
+        return bench_inference([int(c) for c in a.chans.split(",")])
     import torch
     log = lambda m: print(m, file=sys.stderr, flush=True)
     prec = _lib.F16_F32

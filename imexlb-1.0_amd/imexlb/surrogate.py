@@ -270,7 +270,10 @@ class UNetBF16:
                                          1 if f32 else 0, None), "conv3x3_bf16")
 
     def forward(self, x):
-        """x: (N, cin, H, W) FP32 device tensor."""
+        """x: (N, cin, H, W) contiguous FP32 device tensor."""
+        if tuple(x.shape) != (self.N, self.net.cin, self.H, self.W) or x.dtype != self.t.float32 \
+                or not x.is_contiguous():
+            raise ValueError(f"input must be a contiguous float32 ({self.N}, {self.net.cin}, {self.H}, {self.W})")
         L, ch, N = len(self.chans), self.chans, self.N
         _ok(lib().imexlb_nn_pack_input_bf16(_ptr(x), _ptr(self.x), N, self.net.cin, self.cinp, self.H, self.W, None),
             "pack input")

@@ -318,7 +318,8 @@ int imexlb_create(imexlb_ctx **out, int dim, int nx, int ny, int nz, int device,
             (dim == 3 && (rc = dev_alloc(c.get(), (void **)&c->uz, fld))) ||
             (rc = dev_alloc(c.get(), (void **)&c->geo, size_t(c->plane))) ||
             (rc = dev_alloc(c.get(), (void **)&c->rv, 100 * sizeof(double))) ||
-            (rc = dev_alloc(c.get(), (void **)&c->red, 64))) {
+            (rc = dev_alloc(c.get(), (void **)&c->red, 64)) ||
+            (rc = dev_alloc(c.get(), (void **)&c->mass_partial, (MASS_BLOCKS + 1) * sizeof(double)))) {
             c->release();
             return rc;
         }
@@ -791,6 +792,23 @@ int imexlb_forces(imexlb_ctx *c, double F[3])
     F[0] = h[0];
     F[1] = h[1];
     F[2] = c->dim == 3 ? h[2] : 0.0;
+    return IMEXLB_OK;
+}
+
+int imexlb_mass(imexlb_ctx *c, double *m)
+{
+    CHECK_DEV(c);
+    if (!m) return fail(IMEXLB_E_ARG, "null pointer");
+    if (c->stage == S_NONE) return fail(IMEXLB_E_STATE, "no populations yet");
+    int rc = c->materialize();
+    if (rc) return rc;
+    StepArgs a = c->args();
+    a.src = c->f;
+    double *out = c->mass_partial + MASS_BLOCKS;
+    rc = c->launch(2, [&] { return launch_mass(c->dim, c->prec, a, c->mass_partial, out, c->stream); });
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(m, out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return IMEXLB_OK;
 }
 

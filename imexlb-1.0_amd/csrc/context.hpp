@@ -81,6 +81,7 @@ struct imexlb_ctx {
     int8_t *link_q = nullptr;
     int64_t nlink = 0;
     double *force_partial = nullptr, *force_out = nullptr;
+    double *mass_partial = nullptr;  // MASS_BLOCKS partial sums, then the total
     size_t dev_bytes = 0;
 
     // phase state
@@ -230,7 +231,7 @@ struct imexlb_ctx {
         timed.clear();
         void *ptrs[] = {fb[0],     fb[1],  f,         p,        ux,       uy,        uz,     rv,
                         geo,       fill_dst, fill_src, red,     link_out, link_back, link_q, force_partial,
-                        force_out, push_beg, push_q,   push_dst};
+                        force_out, push_beg, push_q,   push_dst, mass_partial};
         for (void *q : ptrs)
             if (q) (void)hipFree(q);
         fb[0] = fb[1] = f = nullptr;
@@ -244,6 +245,7 @@ struct imexlb_ctx {
         link_out = link_back = nullptr;
         link_q = nullptr;
         force_partial = force_out = nullptr;
+        mass_partial = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }

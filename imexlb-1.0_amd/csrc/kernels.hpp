@@ -117,6 +117,10 @@ hipError_t launch_init_up(int dim, const StepArgs &a, const double *rv, hipStrea
 // ps / us, u_x times InitUP's perturbation when rv is given, 0 on bodies
 hipError_t launch_init_given(const StepArgs &a, const double *rv, const float *f, int H, int W, double ps, double us,
                              hipStream_t s);
+// sum over fluid nodes of sum_q f_q of the populations a.src, FP64, into
+// out[0]; partial holds MASS_BLOCKS doubles
+constexpr unsigned MASS_BLOCKS = 1024;
+hipError_t launch_mass(int dim, int prec, const StepArgs &a, double *partial, double *out, hipStream_t s);
 // out[y W + x] = 1 on body nodes, 0 on fluid nodes (2D, y < H, x < W)
 hipError_t launch_body_mask(const StepArgs &a, float *out, int H, int W, hipStream_t s);
 // Boundary fill; the slot tables hold int32 (idx32) or int64 entries.

@@ -732,6 +732,47 @@ __global__ __launch_bounds__(256) void k_monitor(const double *__restrict__ ux, 
 
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
+// Mass check: sum over fluid nodes of sum_q f_q (the density deviation), in
+// FP64 from every storage form.  Each thread adds its nodes (q in order,
+// grid-stride over the nodes), each block writes one partial, and
+// k_sum_partials adds the partials in block order: the result depends on
+// the data only.
+template <int D, class S>
+__global__ __launch_bounds__(256) void k_mass(const S *__restrict__ f, const uint8_t *__restrict__ geo, KParams kp,
+                                              double *__restrict__ partial)
+{
+    using L = Lat<D>;
+    __shared__ double red[4];
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    double v = 0.0;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < kp.N; i += stride) {
+        int x, y, z;
+        uint32_t c;
+        if (!decode<D>(kp, uint32_t(i), x, y, z, c) || !(geo[c] & G_FLUID)) continue;
+        double m = 0.0;
+#pragma unroll
+        for (int q = 0; q < L::Q; ++q) m += double(Store<S>::ld(f[cslot<L::Q>(kp, q, c)]));
+        v += m;
+    }
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = red[0];
+        for (int w = 1; w < int(blockDim.x >> 6); ++w) t += red[w];
+        partial[blockIdx.x] = t;
+    }
+}
+
+__global__ void k_sum_partials(const double *__restrict__ partial, int64_t n, double *__restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    double t = 0.0;
+    for (int64_t b = 0; b < n; ++b) t += partial[b];
+    out[0] = t;
+}
+
 // Momentum exchange on the body links (fluid x -> solid x+e_q) of the
 // post-BoundaryCondition buffer: F += e_q (fb(x,q) + fb(x+e_q, opposite q)),
 // FP64.  Stage 1 writes one partial sum per block; stage 2 adds the partials
@@ -877,6 +918,22 @@ hipError_t launch_init_given(const StepArgs &a, const double *rv, const float *f
 hipError_t launch_body_mask(const StepArgs &a, float *out, int H, int W, hipStream_t s)
 {
     hipLaunchKernelGGL(k_body_mask, dim3(nblocks(a.kp.N)), dim3(256), 0, s, out, a.geo, a.kp, H, W);
+    return hipGetLastError();
+}
+
+hipError_t launch_mass(int dim, int prec, const StepArgs &a, double *partial, double *out, hipStream_t s)
+{
+    const unsigned nb = std::min(nblocks(a.kp.N), MASS_BLOCKS);
+    hipError_t e = dispatch_prec(prec, [&](auto *sp, auto *) {
+        using S = std::remove_pointer_t<decltype(sp)>;
+        if (dim == 2)
+            hipLaunchKernelGGL((k_mass<2, S>), dim3(nb), dim3(256), 0, s, (const S *)a.src, a.geo, a.kp, partial);
+        else
+            hipLaunchKernelGGL((k_mass<3, S>), dim3(nb), dim3(256), 0, s, (const S *)a.src, a.geo, a.kp, partial);
+        return hipGetLastError();
+    });
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, partial, int64_t(nb), out);
     return hipGetLastError();
 }
 

@@ -50,6 +50,7 @@ SIGNATURES = {
     "imexlb_advance": (C.c_int, [_P, C.c_int64]),
     "imexlb_monitor": (C.c_int, [_P, _DP]),
     "imexlb_forces": (C.c_int, [_P, _DP]),
+    "imexlb_mass": (C.c_int, [_P, _DP]),
     "imexlb_download_fields": (C.c_int, [_P, _DP, _DP, C.c_int]),
     "imexlb_download_pdf": (C.c_int, [_P, C.c_int, _DP, C.c_int]),
     "imexlb_synchronize": (C.c_int, [_P]),

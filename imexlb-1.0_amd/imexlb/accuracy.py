@@ -7,9 +7,11 @@ Every form starts from the same state (InitUP with a seeded perturbation)
 and runs the same number of steps; drag and lift are logged every `interv`
 steps (the deck's Monitor interval).  Reported per form: MLUPS, relative L2
 error of the final velocity field against FP64, relative error of the final
-drag and of the mean drag over the second half of the run, and the Strouhal
+drag and of the mean drag over the second half of the run, the Strouhal
 number St = f D / U of the lift signal (its dominant frequency) when the run
-is long enough to show shedding.
+is long enough to show shedding, and the mass check: the mean density
+deviation per node (sum of f over fluid nodes, FP64, imexlb_mass) and its
+difference from the FP64 run.
 """
 from __future__ import annotations
 
@@ -42,8 +44,9 @@ def run_form(deck, prec, steps, every, r_vec):
         done += n
         forces.append(lat.forces())
     _, u = lat.download_fields()
+    mass = lat.mass()
     lat.close()
-    return np.array(forces), u, t_run
+    return np.array(forces), u, t_run, mass
 
 
 def strouhal(lift, every, D, U):
@@ -71,15 +74,17 @@ def study(name, steps, forms, seed=2021):
     ref = None
     for form in forms:
         print(f"[accuracy] {name} {form}: {steps} steps", file=sys.stderr, flush=True)
-        F, u, t = run_form(deck, FORMS[form], steps, every, r_vec)
+        F, u, t, mass = run_form(deck, FORMS[form], steps, every, r_vec)
         half = F[len(F) // 2:]
         res = {"MLUPS": round(nodes(deck) * steps / t / 1e6, 1), "drag_final": float(F[-1, 0]),
+               "density_deviation_per_node": mass / nodes(deck),
                "drag_mean_2nd_half": float(half[:, 0].mean()), "lift_final": float(F[-1, 1]),
                "strouhal": strouhal(F[:, 1], every, D, d.uu)}
         if form == "f64":
-            ref = (F, u)
+            ref = (F, u, mass)
         elif ref is not None:
-            F64, u64 = ref
+            F64, u64, m64 = ref
+            res["density_deviation_per_node_diff"] = float(abs(mass - m64) / nodes(deck))
             res["u_rel_L2_err"] = float(np.linalg.norm(u - u64) / np.linalg.norm(u64))
             res["drag_final_rel_err"] = float(abs(F[-1, 0] - F64[-1, 0]) / abs(F64[-1, 0]))
             m64 = F64[len(F64) // 2:, 0].mean()

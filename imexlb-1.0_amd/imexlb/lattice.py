@@ -150,6 +150,12 @@ class Lattice:
         check(lib().imexlb_monitor(self._h, C.byref(um)), "imexlb_monitor")
         return um.value
 
+    def mass(self):
+        """Sum over fluid nodes of sum_q f_q (FP64)."""
+        m = C.c_double()
+        check(lib().imexlb_mass(self._h, C.byref(m)), "imexlb_mass")
+        return m.value
+
     def forces(self):
         """Momentum-exchange force on the body [drag, lift(, lift_z)], lattice units."""
         F = (C.c_double * 3)()

@@ -173,6 +173,13 @@ int imexlb_monitor(imexlb_ctx *ctx, double *um);
  * Synchronises. */
 int imexlb_forces(imexlb_ctx *ctx, double F[3]);
 
+/* Mass check: sum over fluid nodes of sum_q f_q of the current populations
+ * f (the density deviation; the reference's p is that sum over 3,
+ * lbm.f90:178), accumulated in FP64 from every storage form, in an order
+ * that depends on the lattice only.  No counterpart in the reference.
+ * Synchronises. */
+int imexlb_mass(imexlb_ctx *ctx, double *m);
+
 /* TARGET UPDATE from(u,p) (lbm.f90:244): p and u of the current state. */
 int imexlb_download_fields(imexlb_ctx *ctx, double *p, double *u, int layout);
 /* f (IMEXLB_PDF_F) or fb (IMEXLB_PDF_FB) of the current state, as FP64.

@@ -10,7 +10,7 @@ import pytest
 
 import imexlb as I
 import oracle as O
-from cases import assert_state_equal, get_deck, oracle_setup
+from cases import assert_state_equal, get_deck, interior_mask, oracle_setup
 
 pytestmark = pytest.mark.gpu
 
@@ -262,6 +262,26 @@ def test_forces_match_oracle(name, steps, fused, r_vec):
     scale = np.abs(Fo).max()
     assert scale > 0
     assert np.all(np.abs(Fd - Fo) <= 1e-12 * scale), (Fd, Fo)
+    lat.close()
+    o.close()
+
+
+@pytest.mark.parametrize("name,steps,fused", [("tiny2d", 23, True), ("tiny3d", 9, True), ("wallsphere3d", 7, False)])
+def test_mass_matches_oracle(name, steps, fused, r_vec):
+    """imexlb_mass equals the sum over the oracle's fluid interior nodes of
+    sum_q f_q (FP64; within 1e-12 of the sum of magnitudes, since the
+    summation orders differ), and the device's own f gives the same sum."""
+    lat, o, d = _run_pair(name, r_vec, steps, fused)
+    lat.advance(steps)
+    o.run(0, steps - 1, 0)
+    m = interior_mask(o) & (np.asarray(o.geo) != 0)
+    fo = np.asarray(o.f).reshape(o.A, o.nq)[m]
+    ref, scale = fo.sum(), np.abs(fo).sum()
+    got = lat.mass()
+    assert scale > 0 and abs(got - ref) <= 1e-12 * scale, (got, ref)
+    fd = lat.download_pdf(I.PDF_F, I.LAYOUT_REF).reshape(o.A, o.nq)[m]
+    assert abs(got - fd.sum()) <= 1e-12 * scale
+    assert lat.mass() == got  # the order depends on the lattice only
     lat.close()
     o.close()
 

@@ -174,3 +174,72 @@ def test_bf16_network_matches_numpy():
     assert rel(got, full) < 3e-2, rel(got, full)
     out32 = net.forward(xd).cpu().numpy()
     assert rel(got, out32) < 3e-2
+
+
+def test_trained_network_beats_the_mean_field(monkeypatch):
+    """What the network gains over the method without it, end to end on a
+    256x64 grid: 256 cylinder/ellipse cases (Re 40-150) run by the batched
+    solver to developed fields, a 3-level network trained on the first 224
+    by the library's kernels, then BF16 inference on the other 32.  Its
+    field error is below that of the training set's mean field (the warm
+    start without a network), and the solver started from its predictions
+    reaches the 1 % drag band in fewer steps than a cold start."""
+    import torch
+    import imexlb.surrogate as S
+    monkeypatch.setattr(S, "NX", 256)
+    monkeypatch.setattr(S, "NY", 64)
+    monkeypatch.setattr(S, "RE_RANGE", (40.0, 150.0))
+    print("[surrogate] end-to-end 256x64: data", flush=True)
+    cases = S.make_cases(256, seed=5)
+    X = S.inputs(cases, S.solid_masks(cases))
+    r_vec = np.random.default_rng(5).random(100)
+    Y, _ = S.developed_fields(cases, 8000, r_vec, S._lib.F16_F32, probe_every=20, max_probe_steps=4000)
+    finite = np.isfinite(Y).all(axis=(1, 2, 3))
+    assert finite.all(), np.nonzero(~finite)[0]
+    tr, te = np.arange(224), np.arange(224, 256)
+    net = S.UNet(4, 3, [16, 32, 64], 8, 64, 256, seed=1)
+    Xd, Yd = torch.tensor(X, device="cuda"), torch.tensor(Y, device="cuda")
+    xb = torch.empty((8, 4, 64, 256), device="cuda")
+    yb = torch.empty((8, 3, 64, 256), device="cuda")
+    rng = np.random.default_rng(6)
+    print("[surrogate] end-to-end 256x64: training", flush=True)
+    for step in range(2000):
+        idx = torch.tensor(rng.choice(tr, 8, replace=False), device="cuda")
+        xb.copy_(Xd.index_select(0, idx))
+        yb.copy_(Yd.index_select(0, idx))
+        out = net.forward(xb)
+        net.loss(out, yb)
+        net.backward(net.grad_out)
+        net.adam(2e-3 if step < 1200 else 5e-4)
+    pred = S.UNetBF16(net, len(te)).forward(Xd[te].contiguous()).cpu().numpy()
+    held = Y[te]
+    rel = lambda a: float(np.sqrt(np.mean((a - held) ** 2)) / np.sqrt(np.mean(held ** 2)))
+    err_net, err_mean = rel(pred), rel(Y[tr].mean(axis=0)[None])
+    print(f"[surrogate] held-out rel. RMS error: network {err_net:.4f}, mean field {err_mean:.4f}", flush=True)
+    assert err_net < 0.9 * err_mean, (err_net, err_mean)
+    # the solver started from each: steps until the drag's running mean stays
+    # within 1 % of the cold run's converged value
+    test_cases = [cases[i] for i in te]
+    coords_dev = torch.tensor(S.coords(), device="cuda")
+    eng = S.UNetBF16(net, 16)
+    mean_dev = torch.tensor(np.repeat(Y[tr].mean(axis=0)[None], len(te), axis=0), device="cuda")
+    prec, steps, every, window = S._lib.F16_F32, 24000, 100, 1280
+    print("[surrogate] end-to-end 256x64: cold, network and mean-field starts", flush=True)
+    tc, Fc, *_ = S.run_history(test_cases, steps, every, prec, lambda b: b.init(r_vec))
+    tw, Fw, *_ = S.run_history(test_cases, steps, every, prec,
+                               lambda b: S.warm_start(b, test_cases, eng, coords_dev, r_vec))
+    tm, Fm, *_ = S.run_history(test_cases, steps, every, prec,
+                               lambda b: b.init_fields_f32(mean_dev, S.UU ** 2, S.UU, r_vec))
+    tail = len(tc) // 4
+    res = []
+    for i in range(len(te)):
+        ref = Fc[-tail:, i, 0].mean()
+        res.append([S.convergence_step(t, F[:, i, 0], ref, window) for t, F in ((tc, Fc), (tw, Fw), (tm, Fm))])
+    ok = [r for r in res if all(v is not None for v in r)]
+    med = np.median(np.array(ok, dtype=float), axis=0)
+    print(f"[surrogate] {len(ok)} of {len(res)} cases converged in all three; median steps cold {med[0]:.0f}, "
+          f"network {med[1]:.0f}, mean field {med[2]:.0f}", flush=True)
+    assert len(ok) >= 0.75 * len(res)
+    # at this scale (224 training fields, 2,000 Adam steps) the network start
+    # beats the cold start; the mean-field start is reported, not ranked
+    assert med[1] < med[0]

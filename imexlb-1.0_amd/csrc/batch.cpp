@@ -45,6 +45,7 @@ struct imexlb_batch {
     int block = 256, cpt = 1, persist = 0;
     unsigned persist_grid = 1024;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    bool graphs = true;  // replay captured pairs of steps (IMEXLB_GRAPHS=0: launch every step)
 
     int cur = 1, lazy = -1;
     bool initialized = false, have_pu = false;
@@ -264,6 +265,8 @@ int imexlb_batch_create_shapes(imexlb_batch **out, int ncase, const int32_t *nxn
         b->node_total += int64_t(c->kp.N);
     }
     HIP_TRY(hipSetDevice(device));
+    const char *gv = std::getenv("IMEXLB_GRAPHS");
+    b->graphs = !(gv && gv[0] == '0');
     HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
@@ -418,6 +421,13 @@ int imexlb_batch_advance(imexlb_batch *b, int64_t nsteps)
     CHECK_BATCH(b);
     if (!b->initialized) return fail(IMEXLB_E_STATE, "imexlb_batch_init first");
     if (nsteps < 0) return fail(IMEXLB_E_ARG, "negative step count");
+    if (!b->graphs) {
+        for (int64_t k = 0; k < nsteps; ++k) {
+            int rc = b->one_step();
+            if (rc) return rc;
+        }
+        return IMEXLB_OK;
+    }
     int64_t pairs = nsteps / 2;
     if (pairs > 0) {
         hipGraphExec_t g;
@@ -441,9 +451,16 @@ int imexlb_batch_tune(imexlb_batch *b, int trial_steps, int32_t best[3], double 
     if (!b->initialized) return fail(IMEXLB_E_STATE, "imexlb_batch_init first");
     if (trial_steps < 2) trial_steps = 2;
     trial_steps += trial_steps % 2;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    struct Events {  // destroyed on every return
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events()
+        {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    HIP_TRY(hipEventCreate(&ev.e[0]));
+    HIP_TRY(hipEventCreate(&ev.e[1]));
     double bms = 1e300;
     int bb = b->block, bc = b->cpt, bp = b->persist;
     for (int blk : {64, 128, 256, 512})
@@ -452,12 +469,12 @@ int imexlb_batch_tune(imexlb_batch *b, int trial_steps, int32_t best[3], double 
                 int rc = imexlb_batch_set_config(b, blk, cpt, per);
                 if (!rc) rc = imexlb_batch_advance(b, 2);  // warm-up, builds the graph
                 if (rc) return rc;
-                HIP_TRY(hipEventRecord(e0, b->stream));
+                HIP_TRY(hipEventRecord(ev.e[0], b->stream));
                 if ((rc = imexlb_batch_advance(b, trial_steps))) return rc;
-                HIP_TRY(hipEventRecord(e1, b->stream));
-                HIP_TRY(hipEventSynchronize(e1));
+                HIP_TRY(hipEventRecord(ev.e[1], b->stream));
+                HIP_TRY(hipEventSynchronize(ev.e[1]));
                 float ms = 0.0f;
-                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                HIP_TRY(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
                 const double per_step = double(ms) / trial_steps;
                 if (per_step < bms) {
                     bms = per_step;
@@ -466,8 +483,6 @@ int imexlb_batch_tune(imexlb_batch *b, int trial_steps, int32_t best[3], double 
                     bp = per;
                 }
             }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     int rc = imexlb_batch_set_config(b, bb, bc, bp);
     if (rc) return rc;
     if (best) {

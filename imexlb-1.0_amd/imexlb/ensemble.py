@@ -153,6 +153,9 @@ def main(argv=None):
     ap.add_argument("--seq-steps", type=int, default=2000)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--precision", default="f16")
+    ap.add_argument("--config", default=None,
+                    help="block,cells_per_thread,persistent instead of the autotuner (e.g. 128,2,0)")
+    ap.add_argument("--no-sequential", action="store_true", help="skip the 96 single-lattice runs")
     a = ap.parse_args(argv)
     prec = {"f64": _lib.F64, "f32": _lib.F32, "f16": _lib.F16_F32}[a.precision]
     decks = cylinder_cases(a.cases, a.seed)
@@ -161,8 +164,13 @@ def main(argv=None):
     print(f"[ensemble] {a.cases} cases, {nodes} nodes, {a.precision}", file=sys.stderr, flush=True)
     b = Batch(decks, precision=prec)
     b.init(r_vec)
-    best = b.tune(20)
-    print(f"[ensemble] tuned {best}", file=sys.stderr, flush=True)
+    if a.config:
+        blk, cpt, per = (int(v) for v in a.config.split(","))
+        b.set_config(blk, cpt, bool(per))
+        best = {"block": blk, "cells_per_thread": cpt, "persistent": per, "ms_per_step": None}
+    else:
+        best = b.tune(20)
+    print(f"[ensemble] config {best}", file=sys.stderr, flush=True)
     b.init(r_vec)
     b.synchronize()
     t0 = time.perf_counter()
@@ -171,6 +179,14 @@ def main(argv=None):
     t_batch = time.perf_counter() - t0
     F = b.forces()
     b.close()
+    out = {"cases": a.cases, "nodes": nodes, "precision": a.precision, "tuned": best,
+           "batch_steps": a.steps, "batch_s": round(t_batch, 3),
+           "batch_MLUPS": round(nodes * a.steps / t_batch / 1e6, 1),
+           "nonfinite_cases": int((~np.isfinite(F).all(axis=1)).sum()),
+           "drag_range": [float(np.nanmin(F[:, 0])), float(np.nanmax(F[:, 0]))]}
+    if a.no_sequential:
+        print(json.dumps(out), flush=True)
+        return
     # the same cases as single lattices, one after another
     lats = [setup(d, r_vec, device=0, precision=prec)[0] for d in decks]
     for lat in lats:
@@ -184,13 +200,8 @@ def main(argv=None):
     t_seq = time.perf_counter() - t0
     for lat in lats:
         lat.close()
-    out = {"cases": a.cases, "nodes": nodes, "precision": a.precision, "tuned": best,
-           "batch_steps": a.steps, "batch_s": round(t_batch, 3),
-           "batch_MLUPS": round(nodes * a.steps / t_batch / 1e6, 1),
-           "sequential_steps": a.seq_steps, "sequential_s": round(t_seq, 3),
-           "sequential_MLUPS": round(nodes * a.seq_steps / t_seq / 1e6, 1),
-           "nonfinite_cases": int((~np.isfinite(F).all(axis=1)).sum()),
-           "drag_range": [float(np.nanmin(F[:, 0])), float(np.nanmax(F[:, 0]))]}
+    out.update({"sequential_steps": a.seq_steps, "sequential_s": round(t_seq, 3),
+                "sequential_MLUPS": round(nodes * a.seq_steps / t_seq / 1e6, 1)})
     out["speedup"] = round(out["batch_MLUPS"] / out["sequential_MLUPS"], 2)
     print(json.dumps(out), flush=True)
 

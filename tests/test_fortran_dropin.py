@@ -146,3 +146,20 @@ def test_driver_matches_oracle(case, variant, env, r_vec, tmp_path):
     for k, nm in enumerate(["ux", "uy", "uz"][:deck.dim]):
         assert np.array_equal(plt["data"][nm], u[..., k].ravel() / (d.charlength * d.t_intv))
     o.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,rtol", [("f32", 1e-4), ("f16", 3e-2)])
+def test_driver_storage_forms(prec, rtol, r_vec, tmp_path):
+    """IMEXLB_PRECISION selects a storage form through the drop-in modules:
+    the Monitor series of the 2D deck stays within rtol of the oracle's FP64
+    series."""
+    deck = get_deck("tiny2d")
+    stdout = _run_driver("2d", TINY["tiny2d"][1], str(tmp_path), {"IMEXLB_PRECISION": prec})
+    d = I.derive(deck)
+    o, _ = oracle_setup(deck, r_vec)
+    um = o.run(0, d.max_step, d.interv) / (d.charlength * d.t_intv)
+    printed = np.array([float(v) for v in re.findall(r"Umax=\s*(\S+)", stdout)])
+    assert len(printed) == len(um) > 3
+    assert np.allclose(printed[1:], um[1:], rtol=rtol, atol=0), np.abs(printed / um - 1).max()
+    o.close()

@@ -345,6 +345,10 @@ int imexlb_set_params(imexlb_ctx *c, double ome, double rho0, double uu)
     c->rho0 = c->kp.rho0 = rho0;
     c->kp.inv_rho0 = 1.0 / rho0;
     c->uu = c->kp.uu = uu;
+    c->kp.ome_f = float(ome);
+    c->kp.rho0_f = float(rho0);
+    c->kp.uu_f = float(uu);
+    c->kp.inv_rho0_f = float(c->kp.inv_rho0);
     if (!c->host_only) {
         HIP_TRY(hipSetDevice(c->device));
         HIP_TRY(hipStreamSynchronize(c->stream));

@@ -58,6 +58,7 @@ struct KParams {
     FastDiv div_py;      // / Py
     double inv_rho0;     // 1/rho0 (FP32-compute forms multiply by it)
     uint32_t xcd_remap;  // 1: contiguous runs of blocks per XCD
+    float ome_f, rho0_f, uu_f, inv_rho0_f;  // the same scalars rounded to FP32 once
 };
 
 // slot of direction q at padded cell c, host side (kernels use cslot)

@@ -257,7 +257,7 @@ __device__ __forceinline__ void moments_f32(const float (&f)[Lat<D>::Q], const K
                      sum_where<L::Q>(f, [](int q) { return L::ex(q) < 0; });
     const float sy = sum_where<L::Q>(f, [](int q) { return L::ey(q) > 0; }) -
                      sum_where<L::Q>(f, [](int q) { return L::ey(q) < 0; });
-    const float ir = float(kp.inv_rho0);
+    const float ir = kp.inv_rho0_f;
     p = ps * (1.0f / 3.0f);
     ux = sx * ir;
     uy = sy * ir;
@@ -269,7 +269,7 @@ __device__ __forceinline__ void moments_f32(const float (&f)[Lat<D>::Q], const K
     if (x == 0) {
         const int h = D == 2 ? y : z;
         const int nh = D == 2 ? kp.ny : kp.nz;
-        ux = float(kp.uu) * float(kp.rho0) * 4.0f * float(h) * float(nh - h) / float(nh * nh);
+        ux = kp.uu_f * kp.rho0_f * 4.0f * float(h) * float(nh - h) / float(nh * nh);
         uy = 0.0f;
     }
     if (x == kp.nx) p = 0.0f;
@@ -284,7 +284,7 @@ __device__ __forceinline__ void collide_f32(const float (&f)[Lat<D>::Q], float p
                                             const KParams &kp, Emit emit)
 {
     using L = Lat<D>;
-    const float ome = float(kp.ome), rho0 = float(kp.rho0);
+    const float ome = kp.ome_f, rho0 = kp.rho0_f;
     const float omc = 1.0f - ome;
     float udu = ux * ux;
     udu = __builtin_fmaf(uy, uy, udu);

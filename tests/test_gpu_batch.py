@@ -291,3 +291,35 @@ def test_batch_without_graphs_matches(r_vec, monkeypatch):
         assert np.array_equal(pa, pb) and np.array_equal(ua, ub), i
     a.close()
     b.close()
+
+
+@pytest.mark.gpu
+def test_device_start_and_check_errors(r_vec):
+    """Argument and order errors of the device-memory entry points: a field
+    stack for a 3D lattice, a mask larger than the lattice, the mass before
+    any populations exist, and a stack on a lattice without geometry."""
+    import torch
+    dk = _small_cases()[0]
+    d = I.derive(dk)
+    nx, ny = dk.n[0], dk.n[1]
+    lat = I.Lattice(2, nx, ny, 0, device=0)
+    lat.set_params(dk.ome, dk.rho0, d.uu)
+    stack = torch.zeros((3, ny + 1, nx + 1), device="cuda")
+    with pytest.raises(I.ImexlbError) as e:
+        lat.init_fields_f32(stack, 1.0, 1.0)  # no geometry yet
+    assert e.value.code == -4
+    lat.set_geometry(d.pos, d.radius)
+    with pytest.raises(I.ImexlbError) as e:
+        lat.mass()  # no populations yet
+    assert e.value.code == -4
+    with pytest.raises(I.ImexlbError) as e:
+        lat.body_mask_f32(torch.zeros((ny + 2, nx + 1), device="cuda"))
+    assert e.value.code == -1
+    lat.close()
+    lat3 = I.Lattice(3, 16, 8, 8, device=0)
+    lat3.set_params(1.6, 1.0, 0.02)
+    lat3.set_geometry(8.0, 2.0)
+    with pytest.raises(I.ImexlbError) as e:
+        lat3.init_fields_f32(torch.zeros((3, 9, 17), device="cuda"), 1.0, 1.0)
+    assert e.value.code == -1
+    lat3.close()

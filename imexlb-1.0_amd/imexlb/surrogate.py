@@ -457,7 +457,8 @@ def convergence_step(t, drag, ref, window, tol=0.01):
 def warm_start(b, cases, engine, coords_dev, r_vec):
     """Start the batch `b` of `cases` from the network's prediction, all on
     the GPU: inputs from the solver's geometry, BF16 inference in batches
-    of engine.N, then imexlb_batch_init_fields_f32.  Returns the
+    of engine.N, then imexlb_batch_init_fields_f32 (u_x times InitUP's
+    perturbation with r_vec, unperturbed with None).  Returns the
     predictions (M, 3, NY, NX) as a device tensor."""
     import torch
     M, NB = len(cases), engine.N
@@ -542,6 +543,8 @@ def main(argv=None):
     ap.add_argument("--infer-batch", type=int, default=40)
     ap.add_argument("--eval-steps", type=int, default=40000)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--warm-perturbation", choices=("initup", "none"), default="initup",
+                    help="multiply the predicted u_x by InitUP's perturbation (initup) or start from it as is")
     ap.add_argument("--inference-only", action="store_true",
                     help="time FP32 and BF16 inference of the network (random weights) and stop")
     a = ap.parse_args(argv)
@@ -551,6 +554,7 @@ def main(argv=None):
     log = lambda m: print(m, file=sys.stderr, flush=True)
     prec = _lib.F16_F32
     r_vec = np.random.default_rng(a.seed).random(100)
+    warm_rv = r_vec if a.warm_perturbation == "initup" else None
     cases = make_cases(a.train + a.test, a.seed)
     train, test = cases[:a.train], cases[a.train:]
     masks = solid_masks(cases)
@@ -653,7 +657,7 @@ def main(argv=None):
             fill_inputs(lat, test[:1], x1, coords_dev)
             pred = one.forward(x1)
             torch.cuda.synchronize()
-            lat.init_fields_f32(pred[0], UU ** 2, UU, r_vec)
+            lat.init_fields_f32(pred[0], UU ** 2, UU, warm_rv)
         lat.init_pdf()
         lat.synchronize()
         starts[kind] = time.perf_counter() - t0
@@ -671,7 +675,7 @@ def main(argv=None):
     every = 100
     tc, Fc, sc, ic, _ = run_history(test, a.eval_steps, every, prec, lambda b: b.init(r_vec))
     tw, Fw, sw, iw, preds_dev = run_history(test, a.eval_steps, every, prec,
-                                            lambda b: warm_start(b, test, engine, coords_dev, r_vec))
+                                            lambda b: warm_start(b, test, engine, coords_dev, warm_rv))
     tm, Fm, sm, im, _ = run_history(test, a.eval_steps, every, prec,
                                     lambda b: b.init_fields_f32(mean_dev, UU ** 2, UU, r_vec))
     preds = preds_dev.cpu().numpy()
@@ -713,7 +717,8 @@ def main(argv=None):
     wall_one = [(r["warm"] * t_step_one + starts["warm"]) / (r["cold"] * t_step_one + starts["cold"])
                 for r in valid]
     ratio_m = [r["mean"] / r["cold"] for r in valid if r["mean"]]
-    out = {"train_cases": int(len(tr_idx)), "inputs_built_on_gpu_match": inputs_match, "test_cases": a.test, "params": net.n_params(), "chans": a.chans,
+    out = {"train_cases": int(len(tr_idx)), "seed": a.seed, "warm_perturbation": a.warm_perturbation,
+           "inputs_built_on_gpu_match": inputs_match, "test_cases": a.test, "params": net.n_params(), "chans": a.chans,
            "infer_batch": NB,
            "epochs": a.epochs, "data_s": round(t_data, 1), "train_s": round(t_train, 1),
            "infer_s_per_case": round(t_infer / a.test, 6), "infer_fp32_s_per_case": round(t_infer32 / a.test, 6),

@@ -108,9 +108,9 @@ def test_bf16_pack_pool_upsample_bit_exact():
 
 
 @pytest.mark.parametrize("cip,co,H,W,relu,f32,ostride,ooff", [
-    (16, 32, 20, 36, 1, 0, 32, 0),      # 16-channel chunks, 32-channel blocks, ragged tiles
-    (32, 64, 8, 16, 0, 0, 96, 24),      # one tile, output into a channel slice
-    (96, 128, 13, 50, 1, 0, 128, 0),    # 64-channel blocks, two of them
+    (16, 32, 20, 36, 1, 0, 32, 0),      # one chunk, one channel block, ragged tiles
+    (32, 64, 8, 16, 0, 0, 96, 24),      # one tile, two channel blocks, output into a channel slice
+    (96, 128, 13, 50, 1, 0, 128, 0),    # six chunks, four channel blocks
     (48, 3, 9, 17, 0, 1, 0, 0),         # FP32 NCHW output of 3 channels
     (384, 128, 16, 64, 1, 0, 128, 0),   # the deepest decoder input of the warm-start network
     (64, 48, 24, 40, 1, 0, 48, 0),      # CO not a multiple of 32

@@ -107,8 +107,11 @@ __global__ void k_flip(const float *__restrict__ w, float *__restrict__ wt, int 
 
 constexpr int WCO = 16, WCI = 16;
 
+// partial == nullptr: add to dw with atomics; otherwise write the block's
+// sums to partial[blockIdx.x][co][ci][9] for k_dw_reduce (fixed order).
 __global__ __launch_bounds__(256) void k_conv3_dw(const float *__restrict__ in, const float *__restrict__ dout,
-                                                  float *__restrict__ dw, int N, int CI, int CO, int H, int W)
+                                                  float *__restrict__ dw, int N, int CI, int CO, int H, int W,
+                                                  float *__restrict__ partial)
 {
     __shared__ float tin[WCI][TS + 2][TS + 2];
     __shared__ float tg[WCO][TS * TS];
@@ -146,8 +149,25 @@ __global__ __launch_bounds__(256) void k_conv3_dw(const float *__restrict__ in, 
         __syncthreads();
     }
     if (co0 + co >= CO || ci0 + ci >= CI) return;
+    const int64_t o = (int64_t(co0 + co) * CI + ci0 + ci) * 9;
+    if (partial) {
+        float *pb = partial + int64_t(blockIdx.x) * CO * CI * 9 + o;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) atomicAdd(&dw[(int64_t(co0 + co) * CI + ci0 + ci) * 9 + k], acc[k]);
+        for (int k = 0; k < 9; ++k) pb[k] = acc[k];
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) atomicAdd(&dw[o + k], acc[k]);
+}
+
+// dw[i] += sum over tiles b = 0..nb-1 of partial[b][i], in that order
+__global__ void k_dw_reduce(const float *__restrict__ partial, float *__restrict__ dw, int64_t nb, int64_t n)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int64_t b = 0; b < nb; ++b) s += partial[b * n + i];
+    dw[i] += s;
 }
 
 // db[c] = sum of dout over batch and pixels (one block per channel)
@@ -464,7 +484,26 @@ int imexlb_nn_conv3x3_dw(const float *in, const float *dout, float *dw, int N, i
     if (!in || !dout || !dw || N <= 0 || CI <= 0 || CO <= 0 || H <= 0 || W <= 0) return -1;
     const dim3 grid(unsigned(((W + TS - 1) / TS) * ((H + TS - 1) / TS)),
                     unsigned(((CO + WCO - 1) / WCO) * ((CI + WCI - 1) / WCI)));
-    NN_LAUNCH(k_conv3_dw, grid, dim3(256), 0, (hipStream_t)stream, in, dout, dw, N, CI, CO, H, W);
+    NN_LAUNCH(k_conv3_dw, grid, dim3(256), 0, (hipStream_t)stream, in, dout, dw, N, CI, CO, H, W, nullptr);
+}
+
+int64_t imexlb_nn_conv3x3_dw_work(int CI, int CO, int H, int W)
+{
+    if (CI <= 0 || CO <= 0 || H <= 0 || W <= 0) return -1;
+    return int64_t((W + TS - 1) / TS) * ((H + TS - 1) / TS) * CO * CI * 9;
+}
+
+int imexlb_nn_conv3x3_dw_ordered(const float *in, const float *dout, float *dw, float *work, int64_t work_floats,
+                                 int N, int CI, int CO, int H, int W, void *stream)
+{
+    if (!in || !dout || !dw || !work || N <= 0 || CI <= 0 || CO <= 0 || H <= 0 || W <= 0) return -1;
+    const int64_t tiles = int64_t((W + TS - 1) / TS) * ((H + TS - 1) / TS);
+    const int64_t n = int64_t(CO) * CI * 9;
+    if (work_floats < tiles * n) return -1;
+    const dim3 grid(unsigned(tiles), unsigned(((CO + WCO - 1) / WCO) * ((CI + WCI - 1) / WCI)));
+    hipLaunchKernelGGL(k_conv3_dw, grid, dim3(256), 0, (hipStream_t)stream, in, dout, dw, N, CI, CO, H, W, work);
+    if (hipGetLastError() != hipSuccess) return -2;
+    NN_LAUNCH(k_dw_reduce, dim3(blocks(n)), dim3(256), 0, (hipStream_t)stream, work, dw, tiles, n);
 }
 
 int imexlb_nn_bias_grad(const float *dout, float *db, int N, int C, int H, int W, void *stream)

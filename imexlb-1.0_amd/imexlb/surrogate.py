@@ -116,6 +116,10 @@ class UNet:
         self.dup = [e(self.chans[l + 1], l + 1) for l in range(L - 1)]
         self.inp = [None] * len(self.shapes)
         self.loss_buf = torch.zeros(1, dtype=f32, device=dev)
+        # per-tile weight-gradient sums, added in a fixed order (training repeats bit for bit)
+        work = max(lib().imexlb_nn_conv3x3_dw_work(ci, co, *self.hw[self.level_of[k]])
+                   for k, (co, ci) in enumerate(self.shapes))
+        self.dw_work = torch.empty(int(work), dtype=f32, device=dev)
         self.grad_out = e(cout, 0)
 
     # ---------------------------------------------------------------- layers
@@ -136,8 +140,8 @@ class UNet:
             dz = self.dz[k]
         else:
             dz = g
-        _ok(lib().imexlb_nn_conv3x3_dw(_ptr(self.inp[k]), _ptr(dz), _ptr(self.dw[k]), self.N, ci, co, h, w, None),
-            "conv3x3_dw")
+        _ok(lib().imexlb_nn_conv3x3_dw_ordered(_ptr(self.inp[k]), _ptr(dz), _ptr(self.dw[k]), _ptr(self.dw_work),
+                                               self.dw_work.numel(), self.N, ci, co, h, w, None), "conv3x3_dw")
         _ok(lib().imexlb_nn_bias_grad(_ptr(dz), _ptr(self.db[k]), self.N, co, h, w, None), "bias_grad")
         if not need_dx:
             return None

@@ -23,6 +23,13 @@ int imexlb_nn_flip_weights(const float *w, float *wt, int CI, int CO, void *stre
 /* dw[co,ci,ky,kx] += sum over n, y, x of dout[n,co,y,x] in[n,ci,y+ky-1,x+kx-1] */
 int imexlb_nn_conv3x3_dw(const float *in, const float *dout, float *dw, int N, int CI, int CO, int H, int W,
                          void *stream);
+/* The same sum in an order that depends on the shapes only (results
+ * repeat bit for bit): per-tile sums into work, then added to dw tile by
+ * tile.  work holds imexlb_nn_conv3x3_dw_work(CI, CO, H, W) floats
+ * (work_floats: its size; -1 if too small). */
+int64_t imexlb_nn_conv3x3_dw_work(int CI, int CO, int H, int W);
+int imexlb_nn_conv3x3_dw_ordered(const float *in, const float *dout, float *dw, float *work, int64_t work_floats,
+                                 int N, int CI, int CO, int H, int W, void *stream);
 /* db[c] = sum of dout[:, c] */
 int imexlb_nn_bias_grad(const float *dout, float *db, int N, int C, int H, int W, void *stream);
 /* dz = dout where out > 0, else 0 (out: the ReLU output) */

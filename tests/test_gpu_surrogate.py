@@ -243,3 +243,46 @@ def test_trained_network_beats_the_mean_field(monkeypatch):
     # at this scale (224 training fields, 2,000 Adam steps) the network start
     # beats the cold start; the mean-field start is reported, not ranked
     assert med[1] < med[0]
+
+
+def test_ordered_weight_gradient_repeats_bit_for_bit():
+    """imexlb_nn_conv3x3_dw_ordered gives the atomic version's gradient (to
+    FP32 rounding) and the same bits on every call, and two training runs
+    of the same seed end with identical weights."""
+    import torch
+    L = _lib()
+    rng = np.random.default_rng(41)
+    N, CI, CO, H, W = 3, 24, 40, 37, 70
+    x = torch.tensor(rng.normal(size=(N, CI, H, W)).astype(np.float32), device="cuda")
+    g = torch.tensor(rng.normal(size=(N, CO, H, W)).astype(np.float32), device="cuda")
+    nwork = L.imexlb_nn_conv3x3_dw_work(CI, CO, H, W)
+    work = torch.empty(int(nwork), dtype=torch.float32, device="cuda")
+    dws = []
+    for ordered in (False, True, True):
+        dw = torch.zeros((CO, CI, 3, 3), device="cuda")
+        if ordered:
+            rc = L.imexlb_nn_conv3x3_dw_ordered(_p(x), _p(g), _p(dw), _p(work), nwork, N, CI, CO, H, W, None)
+        else:
+            rc = L.imexlb_nn_conv3x3_dw(_p(x), _p(g), _p(dw), N, CI, CO, H, W, None)
+        assert rc == 0
+        dws.append(dw.cpu().numpy())
+    ref = R.conv3x3_backward(x.cpu().numpy().astype(np.float64), np.zeros((CO, CI, 3, 3)), None,
+                             g.cpu().numpy().astype(np.float64), False)[1]
+    scale = np.abs(ref).max()
+    assert np.abs(dws[0] - ref).max() <= 1e-4 * scale and np.abs(dws[1] - ref).max() <= 1e-4 * scale
+    assert np.array_equal(dws[1], dws[2])
+    assert L.imexlb_nn_conv3x3_dw_ordered(_p(x), _p(g), _p(dw), _p(work), nwork - 1, N, CI, CO, H, W, None) == -1
+
+    def train():
+        net, xs, _ = _setup(chans=(8, 16), N=4, H=16, W=32)
+        xd = torch.tensor(xs, device="cuda")
+        yd = torch.tensor(np.tanh(xs[:, :3]).astype(np.float32), device="cuda")
+        for _ in range(20):
+            out = net.forward(xd)
+            net.loss(out, yd)
+            net.backward(net.grad_out)
+            net.adam(3e-3)
+        return [w.cpu().numpy() for w in net.w]
+
+    a, b = train(), train()
+    assert all(np.array_equal(u, v) for u, v in zip(a, b))

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol(root):
     declared = set()
     for h in ("imexlb.h", "imexlb_nn.h"):
         header = open(os.path.join(root, "include", h)).read()
-        declared |= set(re.findall(r"^\s*(?:int|const char \*)\s*(imexlb_\w+)\s*\(", header, re.M))
+        declared |= set(re.findall(r"^\s*(?:int|int64_t|const char \*)\s*(imexlb_\w+)\s*\(", header, re.M))
     assert len(declared) >= 45
     L = C.CDLL(_lib.LIB_PATH)
     for name in sorted(declared):

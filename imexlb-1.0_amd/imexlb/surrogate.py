@@ -1,6 +1,6 @@
 """Learned warm start for D2Q9 cylinder/ellipse cases.
 
-    python -m imexlb.surrogate [--train 2000] [--test 200] [--epochs 20]
+    python -m imexlb.surrogate [--train 2000] [--test 200] [--epochs 30]
     python -m imexlb.surrogate --inference-only
 
 1. Cases: 512x128 channel, a body of the cylinder's area (radius Ny/10) at
@@ -540,7 +540,7 @@ def main(argv=None):
     ap.add_argument("--train", type=int, default=2000)
     ap.add_argument("--test", type=int, default=200)
     ap.add_argument("--data-steps", type=int, default=24000)
-    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--chans", default="32,64,128,256")

@@ -11,7 +11,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <memory>
+#include <string>
 #include <new>
 #include <vector>
 
@@ -36,6 +38,8 @@ struct imexlb_batch {
     int64_t *link_out = nullptr, *link_back = nullptr;
     int8_t *link_q = nullptr;
     double *force_partial = nullptr, *force_out = nullptr;
+    unsigned long long *mon = nullptr;  // Monitor's two result words per case
+    std::vector<uint32_t> skip;         // per case: the fluid node Monitor leaves out
     BatchCase *d_cases = nullptr;
     uint32_t *queue = nullptr;
     uint32_t *tile_case = nullptr, *tile_first = nullptr;
@@ -66,9 +70,9 @@ struct imexlb_batch {
             (void)hipStreamSynchronize(stream);
         }
         drop_graphs();
-        void *ptrs[] = {fb[0],     fb[1],  f,           p,         ux,        uy,     rv,   geo,       fill_dst,
-                        fill_src,  link_out, link_back, link_q,    force_partial, force_out, d_cases, queue,
-                        tile_case, tile_first};
+        void *ptrs[] = {fb[0],     fb[1],      f,        p,         ux,        uy,            rv,
+                        geo,       fill_dst,   fill_src, link_out,  link_back, link_q,        force_partial,
+                        force_out, mon,        d_cases,  queue,     tile_case, tile_first};
         for (void *q : ptrs)
             if (q) (void)hipFree(q);
         if (stream) (void)hipStreamDestroy(stream);
@@ -240,6 +244,7 @@ int imexlb_batch_create_shapes(imexlb_batch **out, int ncase, const int32_t *nxn
             return rc;
         DeviceTables t;
         if ((rc = build_device_tables(c, t))) return rc;
+        b->skip.push_back(t.skip_c);
         BatchCase bc;
         bc.kp = c->kp;
         bc.pop_base = b->pop_total;
@@ -288,6 +293,7 @@ int imexlb_batch_create_shapes(imexlb_batch **out, int ncase, const int32_t *nxn
         (rc = balloc(B, (void **)&B->link_q, nl)) ||
         (rc = balloc(B, (void **)&B->force_partial, nblk * 3 * sizeof(double))) ||
         (rc = balloc(B, (void **)&B->force_out, size_t(3 * ncase) * sizeof(double))) ||
+        (rc = balloc(B, (void **)&B->mon, size_t(2 * ncase) * sizeof(unsigned long long))) ||
         (rc = balloc(B, (void **)&B->d_cases, size_t(ncase) * sizeof(BatchCase))) ||
         (rc = balloc(B, (void **)&B->queue, 64)))
         return rc;
@@ -517,6 +523,36 @@ int imexlb_batch_forces(imexlb_batch *b, double *F)
     HIP_TRY(hipMemcpyAsync(h.data(), b->force_out, h.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     std::memcpy(F, h.data(), h.size() * sizeof(double));
+    return IMEXLB_OK;
+}
+
+int imexlb_batch_monitor(imexlb_batch *b, double *um)
+{
+    CHECK_BATCH(b);
+    if (!um) return fail(IMEXLB_E_ARG, "null pointer");
+    if (!b->initialized) return fail(IMEXLB_E_STATE, "imexlb_batch_init first");
+    for (int i = 0; i < b->ncase; ++i) {
+        int rc = b->materialize(i);
+        if (rc) return rc;
+        hipError_t e = launch_monitor(2, b->case_args(i), b->skip[size_t(i)], b->mon + 2 * i, b->stream);
+        if (e != hipSuccess) return fail(IMEXLB_E_HIP, std::string("batch monitor: ") + hipGetErrorString(e));
+    }
+    std::vector<unsigned long long> h(size_t(2 * b->ncase));
+    HIP_TRY(hipMemcpyAsync(h.data(), b->mon, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    int diverged = 0;
+    for (int i = 0; i < b->ncase; ++i) {
+        if (h[size_t(2 * i + 1)] != 0) {
+            um[i] = std::numeric_limits<double>::quiet_NaN();
+            ++diverged;
+            continue;
+        }
+        double m;
+        std::memcpy(&m, &h[size_t(2 * i)], sizeof(m));
+        um[i] = std::sqrt(m);
+    }
+    if (diverged) return fail(IMEXLB_E_NAN, std::to_string(diverged) + " case(s) with a non-finite velocity");
     return IMEXLB_OK;
 }
 

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <new>
 #include <string>
@@ -221,6 +222,7 @@ const char *imexlb_strerror(int code)
     case IMEXLB_E_STATE: return "call out of the Collision..PostProcessing order";
     case IMEXLB_E_UB: return "result depends on a read past the end of the reference arrays";
     case IMEXLB_E_NODEV: return "compute call on a host-only context";
+    case IMEXLB_E_NAN: return "non-finite velocity (the lattice has diverged)";
     default: return "unknown error";
     }
 }
@@ -253,8 +255,13 @@ int imexlb_create(imexlb_ctx **out, int dim, int nx, int ny, int nz, int device,
     c->g.init(dim, nx, ny, dim == 3 ? nz : 0);
     if (c->g.A >= (int64_t(1) << 31)) return fail(IMEXLB_E_ARG, "array_size exceeds the reference's 32-bit ids");
 
-    // padded device layout, one ghost layer
-    c->Px = int64_t(nx) + 3;
+    // padded device layout, one ghost layer; rows optionally padded to a
+    // multiple of `align` cells (IMEXLB_ROW_ALIGN) so that the y- and
+    // z-shifted pulls of a tile start on a tile boundary
+    int64_t align = 1;
+    if (const char *av = std::getenv("IMEXLB_ROW_ALIGN")) align = std::max<int64_t>(1, std::atoll(av));
+    if (align > 4096) return fail(IMEXLB_E_ARG, "IMEXLB_ROW_ALIGN above 4096 cells");
+    c->Px = ((int64_t(nx) + 3 + align - 1) / align) * align;
     c->Py = int64_t(ny) + 3;
     c->Pz = dim == 3 ? int64_t(nz) + 3 : 1;
     c->PxPy = c->Px * c->Py;
@@ -290,6 +297,7 @@ int imexlb_create(imexlb_ctx **out, int dim, int nx, int ny, int nz, int device,
     for (uint32_t n : {0u, uint32_t(c->Px), k.ncell - 1u, 0xfffffffeu})
         if (k.div_px.host_div(n) != n / uint32_t(c->Px) || k.div_py.host_div(n) != n / uint32_t(c->Py))
             return fail(IMEXLB_E_ARG, "fast division self-check failed");
+    k.back_tiles = uint32_t((1 + k.Px + k.PxPy + 63) / 64);
     k.rdy = c->g.dy;
     k.rdz = c->g.dz;
     // XCD-contiguous block order pays for D3Q27 FP64 only (measured on
@@ -297,6 +305,7 @@ int imexlb_create(imexlb_ctx **out, int dim, int nx, int ny, int nz, int device,
     const char *xv = std::getenv("IMEXLB_XCD_REMAP");
     k.xcd_remap = xv ? uint32_t(xv[0] == '1') : uint32_t(dim == 3 && precision == IMEXLB_F64);
 
+    if (const char *fv = std::getenv("IMEXLB_FAIL_ALLOC")) c->fail_alloc_at = std::atoll(fv);
     if (device >= 0) {
         int ndev = 0;
         HIP_TRY(hipGetDeviceCount(&ndev));
@@ -369,15 +378,30 @@ int imexlb_set_geometry_shape(imexlb_ctx *c, double pos, double radius, double a
         return fail(IMEXLB_E_ARG, "non-finite geometry");
     if (aspect <= 0.0) return fail(IMEXLB_E_ARG, "ellipse axis ratio must be positive");
     if (c->dim == 3 && (aspect != 1.0 || angle_deg != 0.0)) return fail(IMEXLB_E_ARG, "ellipses are 2D");
+    if (!c->host_only) {
+        HIP_TRY(hipSetDevice(c->device));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    // From here on the old geometry is gone: every return before the end
+    // leaves the context without one (forget_geometry), so later calls get
+    // IMEXLB_E_STATE instead of running on half-built tables.
+    c->forget_geometry();
+    struct Commit {
+        imexlb_ctx *c;
+        bool done = false;
+        ~Commit()
+        {
+            if (!done) c->forget_geometry();
+        }
+    } commit{c};
     c->g.build(pos, radius, aspect, angle_deg);
     c->bc = resolve_boundary(c->g);
-    c->geom_set = true;
     if (c->bc.undefined) return fail(IMEXLB_E_UB, c->bc.error);
-    if (c->host_only) return IMEXLB_OK;
+    if (c->host_only) {
+        commit.done = c->geom_set = true;
+        return IMEXLB_OK;
+    }
 
-    HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    c->drop_graphs();
     DeviceTables t;
     int rc = build_device_tables(c, t);
     if (rc) return rc;
@@ -425,9 +449,7 @@ int imexlb_set_geometry_shape(imexlb_ctx *c, double pos, double radius, double a
         HIP_TRY(hipMemcpy(c->link_back, t.link_back.data(), t.link_back.size() * 8, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(c->link_q, t.link_q.data(), t.link_q.size(), hipMemcpyHostToDevice));
     }
-    c->stage = S_NONE;
-    c->have_pu = c->have_f = c->have_fb = false;
-    c->lazy_src = -1;
+    commit.done = c->geom_set = true;
     return IMEXLB_OK;
 }
 
@@ -764,11 +786,15 @@ int imexlb_monitor(imexlb_ctx *c, double *um)
     StepArgs a = c->args();
     rc = c->launch(2, [&] { return launch_monitor(c->dim, a, c->skip_c, c->red, c->stream); });
     if (rc) return rc;
-    unsigned long long bits = 0;
-    HIP_TRY(hipMemcpyAsync(&bits, c->red, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+    unsigned long long words[2] = {0, 0};  // maximum of ux^2+uy^2, non-finite count
+    HIP_TRY(hipMemcpyAsync(words, c->red, sizeof(words), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (words[1] != 0) {
+        *um = std::numeric_limits<double>::quiet_NaN();
+        return fail(IMEXLB_E_NAN, std::to_string(words[1]) + " fluid node(s) with a non-finite velocity");
+    }
     double m;
-    std::memcpy(&m, &bits, sizeof(m));
+    std::memcpy(&m, &words[0], sizeof(m));
     *um = std::sqrt(m);
     return IMEXLB_OK;
 }

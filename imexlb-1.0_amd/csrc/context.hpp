@@ -83,6 +83,23 @@ struct imexlb_ctx {
     double *force_partial = nullptr, *force_out = nullptr;
     double *mass_partial = nullptr;  // MASS_BLOCKS partial sums, then the total
     size_t dev_bytes = 0;
+    // test hook: IMEXLB_FAIL_ALLOC=N (read at creation) makes this
+    // context's N-th device allocation fail with IMEXLB_E_OOM
+    int64_t alloc_calls = 0, fail_alloc_at = 0;
+
+    // Forget the geometry after a failed SetGeometry: every call that needs
+    // one (InitUP, the phases, Monitor, downloads) then returns
+    // IMEXLB_E_STATE until SetGeometry succeeds.
+    void forget_geometry()
+    {
+        geom_set = false;
+        nfill = nlink = 0;
+        push_on = prefilled = false;
+        stage = S_NONE;
+        have_pu = have_f = have_fb = false;
+        lazy_src = -1;
+        drop_graphs();
+    }
 
     // phase state
     Stage stage = S_NONE;
@@ -255,6 +272,10 @@ namespace imexlb {
 
 inline int dev_alloc(imexlb_ctx *c, void **ptr, size_t bytes)
 {
+    if (c->fail_alloc_at > 0 && ++c->alloc_calls == c->fail_alloc_at) {
+        *ptr = nullptr;
+        return fail(IMEXLB_E_OOM, "hipMalloc(" + std::to_string(bytes) + "): failure injected by IMEXLB_FAIL_ALLOC");
+    }
     hipError_t e = hipMalloc(ptr, bytes);
     if (e != hipSuccess) {
         *ptr = nullptr;

@@ -59,14 +59,29 @@ struct KParams {
     double inv_rho0;     // 1/rho0 (FP32-compute forms multiply by it)
     uint32_t xcd_remap;  // 1: contiguous runs of blocks per XCD
     float ome_f, rho0_f, uu_f, inv_rho0_f;  // the same scalars rounded to FP32 once
+    uint32_t back_tiles; // ceil((1 + Px + PxPy) / 64): the farthest tile a pull reads behind its own
 };
+
+// Position of direction q inside a tile.  IMEXLB_QGROUP = 1 stores the
+// directions that pull from the same source row (equal e_y, e_z) next to
+// each other, e_x fastest, so a wave reads each source tile as one run;
+// 0 keeps the lattice numbering (initialization.f90:112-123).
+#ifndef IMEXLB_QGROUP
+#define IMEXLB_QGROUP 0
+#endif
+__host__ __device__ constexpr int store_q(int q, int Q)
+{
+    return !IMEXLB_QGROUP ? q
+           : Q == 9       ? (q % 3) * 3 + q / 3
+                          : ((q / 9) * 3 + q % 3) * 3 + (q % 9) / 3;
+}
 
 // slot of direction q at padded cell c, host side (kernels use cslot)
 inline int64_t pop_slot(int64_t q, int64_t c, int64_t plane, int Q)
 {
 #if IMEXLB_TILE
     (void)plane;
-    return ((c / IMEXLB_TILE) * Q + q) * IMEXLB_TILE + c % IMEXLB_TILE;
+    return ((c / IMEXLB_TILE) * Q + store_q(int(q), Q)) * IMEXLB_TILE + c % IMEXLB_TILE;
 #else
     (void)Q;
     return q * plane + c;

@@ -44,6 +44,10 @@
 #ifndef IMEXLB_STEP_WAVES
 #define IMEXLB_STEP_WAVES 0
 #endif
+// cells per thread of the fused collide step (1, 2 or 4)
+#ifndef IMEXLB_STEP_CPT
+#define IMEXLB_STEP_CPT 1
+#endif
 
 namespace imexlb {
 
@@ -211,7 +215,7 @@ __device__ __forceinline__ uint64_t cslot(const KParams &kp, uint32_t q, uint32_
 {
 #if IMEXLB_TILE
     constexpr uint32_t T = IMEXLB_TILE;
-    return uint64_t(cell / T) * (uint64_t(Q) * T) + ((cell % T) | (q * T));
+    return uint64_t(cell / T) * (uint64_t(Q) * T) + ((cell % T) | (uint32_t(store_q(int(q), Q)) * T));
 #else
     return uint64_t(q) * uint64_t(kp.plane) + cell;
 #endif
@@ -416,6 +420,71 @@ __device__ __forceinline__ uint64_t push_slot(const PushTable &pt, int32_t e)
                             : uint64_t(static_cast<const int32_t *>(pt.dst)[e]);
 }
 
+// The gather half of a node update: the Q populations the node pulls (or
+// its own with K_NOSHIFT).  Every gathered slot exists in the arrays also
+// for solid nodes, so the loads need not wait for the node's type.
+template <int D, class S, class C, int MODE>
+__device__ __forceinline__ void gather(const S *__restrict__ src, const KParams &kp, uint32_t c,
+                                       C (&f)[Lat<D>::Q])
+{
+    using L = Lat<D>;
+#pragma unroll
+    for (int j = 0; j < L::Q; ++j) {
+        const int q = load_order<D>(j);
+        const S *a = src + cslot<L::Q>(kp, q, (MODE & K_NOSHIFT) ? c : c - shift32<D>(kp, q));
+        f[q] = Store<S>::ld(IMEXLB_STEP_NTLOAD ? nt_load(a) : *a);
+    }
+}
+
+// Internal mode bit of the step kernel: the wave covers exactly one 64-cell
+// tile (cell-mapped threads, 64-cell tiles, one cell per thread), so slots
+// are formed from a wave-uniform tile base plus a 32-bit lane offset.
+constexpr int K_WAVE = 64;
+#ifndef IMEXLB_ADDR32
+#define IMEXLB_ADDR32 0
+#endif
+constexpr bool kWaveSlots = IMEXLB_ADDR32 && IMEXLB_CELLMAP && IMEXLB_TILE == 64;
+
+// the tile of the wave that holds cell c (every lane of the wave has it)
+__device__ __forceinline__ uint32_t wave_tile(uint32_t c) { return __builtin_amdgcn_readfirstlane(c >> 6); }
+
+// gather() for K_WAVE: the populations the node pulls, addressed from the
+// tile back_tiles behind the wave's own, the lowest any pull can reach, so
+// every offset is a non-negative 32-bit count of values.
+template <int D, class S, class C>
+__device__ __forceinline__ void gather_wave(const S *__restrict__ src, const KParams &kp, uint32_t c,
+                                            C (&f)[Lat<D>::Q])
+{
+    using L = Lat<D>;
+    constexpr uint32_t QT = uint32_t(L::Q) * 64u;
+    const uint32_t back = kp.back_tiles;
+    // near the start of the array the base lies before it; only base + off
+    // is ever read (pointer arithmetic keeps the global address space)
+    const S *base = src + (int64_t(wave_tile(c)) - int64_t(back)) * int64_t(QT);
+    const uint32_t lane = c & 63u;
+#pragma unroll
+    for (int j = 0; j < L::Q; ++j) {
+        const int q = load_order<D>(j);
+        // source cell relative to the start of the base tile; byte offsets
+        // in 32 bits (tile count < 2^24 and the product < 2^32 for any
+        // lattice whose slots fit 32-bit cells), the direction's position
+        // in the tile as a constant added last
+        const uint32_t rel = lane + back * 64u - shift32<D>(kp, q);
+        const uint32_t boff = __umul24(rel >> 6, QT * uint32_t(sizeof(S))) + (rel & 63u) * uint32_t(sizeof(S));
+        const S *a = reinterpret_cast<const S *>(reinterpret_cast<const char *>(base) + boff) +
+                     uint32_t(store_q(q, L::Q)) * 64u;
+        f[q] = Store<S>::ld(IMEXLB_STEP_NTLOAD ? nt_load(a) : *a);
+    }
+}
+
+// The rest of a node update from its gathered populations and node type g.
+template <int D, class S, class C, int MODE>
+__device__ __forceinline__ void finish_node(const C (&f)[Lat<D>::Q], uint8_t g, const S *__restrict__ src,
+                                            S *__restrict__ dst, S *__restrict__ fout, double *__restrict__ pout,
+                                            double *__restrict__ uxo, double *__restrict__ uyo,
+                                            double *__restrict__ uzo, const KParams &kp, int x, int y, int z,
+                                            uint32_t c, const PushTable &pt);
+
 template <int D, class S, class C, int MODE>
 __device__ __forceinline__ void step_node(const S *__restrict__ src, S *__restrict__ dst, S *__restrict__ fout,
                                           double *__restrict__ pout, double *__restrict__ uxo,
@@ -423,19 +492,22 @@ __device__ __forceinline__ void step_node(const S *__restrict__ src, S *__restri
                                           const uint8_t *__restrict__ geo, const KParams &kp, int x, int y, int z,
                                           uint32_t c, const PushTable &pt = PushTable{})
 {
-    using L = Lat<D>;
-    // The node's type is loaded together with its populations, not before
-    // them, so the gather does not wait for it (every gathered slot is in
-    // the arrays also for solid nodes).
+    // the node's type is loaded together with its populations, not before
     const uint8_t g = geo[c];
+    C f[Lat<D>::Q];
+    gather<D, S, C, MODE>(src, kp, c, f);
+    finish_node<D, S, C, MODE>(f, g, src, dst, fout, pout, uxo, uyo, uzo, kp, x, y, z, c, pt);
+}
+
+template <int D, class S, class C, int MODE>
+__device__ __forceinline__ void finish_node(const C (&f)[Lat<D>::Q], uint8_t g, const S *__restrict__ src,
+                                            S *__restrict__ dst, S *__restrict__ fout, double *__restrict__ pout,
+                                            double *__restrict__ uxo, double *__restrict__ uyo,
+                                            double *__restrict__ uzo, const KParams &kp, int x, int y, int z,
+                                            uint32_t c, const PushTable &pt)
+{
+    using L = Lat<D>;
     const bool fluid = (g & G_FLUID) != 0;
-    C f[L::Q];
-#pragma unroll
-    for (int j = 0; j < L::Q; ++j) {
-        const int q = load_order<D>(j);
-        const S *a = src + cslot<L::Q>(kp, q, (MODE & K_NOSHIFT) ? c : c - shift32<D>(kp, q));
-        f[q] = Store<S>::ld(IMEXLB_STEP_NTLOAD ? nt_load(a) : *a);
-    }
     if (!fluid) {
         if (MODE & K_WRITE_PU) {  // PostProcessing's inlet/outlet loops also reach solid nodes
             if (x == 0) {
@@ -466,8 +538,11 @@ __device__ __forceinline__ void step_node(const S *__restrict__ src, S *__restri
         if (D == 3) uzo[c] = double(uz);
     }
     if (MODE & K_COLLIDE) {
+        // K_WAVE: the wave is one tile; its output slots are a scalar tile
+        // base plus the lane
+        S *const wdst = (MODE & K_WAVE) ? dst + uint64_t(wave_tile(c)) * (L::Q * 64u) : dst;
         collide<D, C>(f, p, ux, uy, uz, kp, [&](int q, C v) {
-            S *o = dst + cslot<L::Q>(kp, q, c);
+            S *o = (MODE & K_WAVE) ? wdst + (uint32_t(store_q(q, L::Q)) * 64u + (c & 63u)) : dst + cslot<L::Q>(kp, q, c);
             if (IMEXLB_STEP_NT)
                 nt_store(o, Store<S>::template st<C>(v));
             else
@@ -486,6 +561,15 @@ __device__ __forceinline__ void step_node(const S *__restrict__ src, S *__restri
 #define IMEXLB_STEP_BOUNDS __launch_bounds__(IMEXLB_STEP_BLOCK)
 #endif
 
+// Cells per thread of the fused collide step: a wave takes CPT consecutive
+// 64-cell tiles, lane l cell l of each, and issues the loads of all of them
+// before any arithmetic (more bytes in flight per wave).
+template <int MODE>
+__host__ __device__ constexpr int step_cpt()
+{
+    return (MODE & K_COLLIDE) && !(MODE & (K_WRITE_F | K_WRITE_PU)) ? IMEXLB_STEP_CPT : 1;
+}
+
 template <int D, class S, class C, int MODE>
 __global__ IMEXLB_STEP_BOUNDS void k_step(const S *__restrict__ src, S *__restrict__ dst,
                                               S *__restrict__ fout, double *__restrict__ pout,
@@ -493,11 +577,44 @@ __global__ IMEXLB_STEP_BOUNDS void k_step(const S *__restrict__ src, S *__restri
                                               double *__restrict__ uzo, const uint8_t *__restrict__ geo,
                                               KParams kp, PushTable pt)
 {
-    int x, y, z;
-    uint32_t c;
-    const uint32_t i = xcd_block(blockIdx.x, gridDim.x, kp.xcd_remap != 0) * blockDim.x + threadIdx.x;
-    if (!node_of<D>(kp, i, x, y, z, c)) return;
-    step_node<D, S, C, MODE>(src, dst, fout, pout, uxo, uyo, uzo, geo, kp, x, y, z, c, pt);
+    constexpr int CPT = step_cpt<MODE>();
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, kp.xcd_remap != 0);
+    if constexpr (CPT == 1) {
+        int x, y, z;
+        uint32_t c;
+        const uint32_t i = blk * blockDim.x + threadIdx.x;
+        if (!node_of<D>(kp, i, x, y, z, c)) return;
+        if constexpr (kWaveSlots && MODE == (MODE & (K_COLLIDE | K_PUSH | K_IDX64))) {
+            const uint8_t g = geo[c];
+            C f[Lat<D>::Q];
+            gather_wave<D, S, C>(src, kp, c, f);
+            finish_node<D, S, C, MODE | K_WAVE>(f, g, src, dst, fout, pout, uxo, uyo, uzo, kp, x, y, z, c, pt);
+        } else {
+            step_node<D, S, C, MODE>(src, dst, fout, pout, uxo, uyo, uzo, geo, kp, x, y, z, c, pt);
+        }
+    } else {
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint32_t wave = (blk * blockDim.x + threadIdx.x) >> 6;
+        int x[CPT], y[CPT], z[CPT];
+        uint32_t c[CPT];
+        bool ok[CPT];
+        uint8_t g[CPT];
+        C f[CPT][Lat<D>::Q];
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            ok[k] = node_of<D>(kp, (wave * CPT + k) * 64u + lane, x[k], y[k], z[k], c[k]);
+            g[k] = 0;
+            if (ok[k]) {
+                g[k] = geo[c[k]];
+                gather<D, S, C, MODE>(src, kp, c[k], f[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < CPT; ++k)
+            if (ok[k])
+                finish_node<D, S, C, MODE>(f[k], g[k], src, dst, fout, pout, uxo, uyo, uzo, kp, x[k], y[k], z[k],
+                                           c[k], pt);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -692,7 +809,9 @@ __global__ __launch_bounds__(256) void k_fill(S *__restrict__ buf, const S *__re
 // the maximum of the roots equals the root of the maximum).  A bounded grid
 // strides over the nodes with MON_UNROLL independent loads in flight per
 // thread; one 64-bit atomicMax per block (non-negative doubles order like
-// their bit patterns).
+// their bit patterns) into out[0].  Nodes where ux^2+uy^2 is not finite
+// (NaN or infinite: a diverged lattice) are left out of the maximum and
+// counted into out[1], one atomicAdd per block that found any.
 constexpr unsigned MON_BLOCKS = 1024;
 constexpr int MON_UNROLL = 4;
 
@@ -702,8 +821,12 @@ __global__ __launch_bounds__(256) void k_monitor(const double *__restrict__ ux, 
                                                  unsigned long long *__restrict__ out)
 {
     __shared__ double red[4];
+    __shared__ unsigned bad_blk;
+    if (threadIdx.x == 0) bad_blk = 0u;
+    __syncthreads();
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     double v = 0.0;
+    unsigned bad = 0u;
     for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; base < kp.N;
          base += MON_UNROLL * stride) {
         double a[MON_UNROLL];
@@ -717,20 +840,35 @@ __global__ __launch_bounds__(256) void k_monitor(const double *__restrict__ ux, 
                 a[k] = ux[c] * ux[c] + uy[c] * uy[c];
         }
 #pragma unroll
-        for (int k = 0; k < MON_UNROLL; ++k) v = fmax(v, a[k]);
+        for (int k = 0; k < MON_UNROLL; ++k) {
+            const bool fin = __builtin_isfinite(a[k]);
+            bad += fin ? 0u : 1u;
+            v = fin ? fmax(v, a[k]) : v;
+        }
     }
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    for (int o = 32; o > 0; o >>= 1) {
+        v = fmax(v, __shfl_xor(v, o, 64));
+        bad += __shfl_xor(bad, o, 64);
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) red[wave] = v;
+    if (lane == 0) {
+        red[wave] = v;
+        if (bad) atomicAdd(&bad_blk, bad);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = red[0];
         for (int w = 1; w < int(blockDim.x >> 6); ++w) m = fmax(m, red[w]);
         atomicMax(out, (unsigned long long)__double_as_longlong(m));
+        if (bad_blk) atomicAdd(out + 1, (unsigned long long)bad_blk);
     }
 }
 
-__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+// clears the Monitor's two result words (maximum, non-finite count)
+__global__ void k_zero_u64(unsigned long long *p)
+{
+    if (threadIdx.x < 2) p[threadIdx.x] = 0ull;
+}
 
 // Mass check: sum over fluid nodes of sum_q f_q (the density deviation), in
 // FP64 from every storage form.  Each thread adds its nodes (q in order,
@@ -840,7 +978,8 @@ static hipError_t step_dm(int prec, const StepArgs &a, hipStream_t s)
     return dispatch_prec(prec, [&](auto *sp, auto *cp) {
         using S = std::remove_pointer_t<decltype(sp)>;
         using C = std::remove_pointer_t<decltype(cp)>;
-        const unsigned nb = unsigned((node_threads(a.kp) + IMEXLB_STEP_BLOCK - 1) / IMEXLB_STEP_BLOCK);
+        constexpr uint64_t per_block = uint64_t(IMEXLB_STEP_BLOCK) * step_cpt<MODE>();
+        const unsigned nb = unsigned((node_threads(a.kp) + per_block - 1) / per_block);
         hipLaunchKernelGGL((k_step<D, S, C, MODE>), dim3(nb), dim3(IMEXLB_STEP_BLOCK), 0, s, (const S *)a.src,
                            (S *)a.dst, (S *)a.fout, a.p, a.ux, a.uy, a.uz, a.geo, a.kp, a.push);
         return hipGetLastError();
@@ -974,7 +1113,7 @@ hipError_t launch_forces(int dim, int prec, const void *buf, const int64_t *out_
 
 hipError_t launch_monitor(int dim, const StepArgs &a, uint32_t skip, unsigned long long *out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, out);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, out);
     const unsigned nb = std::min(nblocks(a.kp.N), MON_BLOCKS);
     if (dim == 2)
         hipLaunchKernelGGL((k_monitor<2>), dim3(nb), dim3(256), 0, s, a.ux, a.uy, a.geo, skip, a.kp, out);

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_PKG_ROOT, "libimexlb.so")
 _LIB = None
 
 OK = 0
-E_ARG, E_HIP, E_OOM, E_STATE, E_UB, E_NODEV = -1, -2, -3, -4, -5, -6
+E_ARG, E_HIP, E_OOM, E_STATE, E_UB, E_NODEV, E_NAN = -1, -2, -3, -4, -5, -6, -7
 F64, F32, F16_F32, BF16_F32 = 0, 1, 2, 3
 LAYOUT_REF, LAYOUT_INTERIOR = 0, 1
 PDF_F, PDF_FB = 0, 1
@@ -69,6 +69,7 @@ SIGNATURES = {
     "imexlb_batch_tune": (C.c_int, [_P, C.c_int, _I32P, _DP]),
     "imexlb_batch_synchronize": (C.c_int, [_P]),
     "imexlb_batch_forces": (C.c_int, [_P, _DP]),
+    "imexlb_batch_monitor": (C.c_int, [_P, _DP]),
     "imexlb_batch_download_fields": (C.c_int, [_P, C.c_int, _DP, _DP]),
     "imexlb_batch_info": (C.c_int, [_P, _I64P]),
     # warm-start network kernels (include/imexlb_nn.h); float pointers are device addresses

@@ -4,12 +4,14 @@
     python -m imexlb.accuracy cyl8192x2048 [--steps 50000]
 
 Every form starts from the same state (InitUP with a seeded perturbation)
-and runs the same number of steps; drag and lift are logged every `interv`
-steps (the deck's Monitor interval).  Reported per form: MLUPS, relative L2
-error of the final velocity field against FP64, relative error of the final
+and runs the same number of steps; drag and lift are logged every 10 steps
+(--log-every; at most the deck's Monitor interval).  Reported per form:
+MLUPS, relative L2 error of the final velocity field against FP64, relative
+error of the final
 drag and of the mean drag over the second half of the run, the Strouhal
-number St = f D / U of the lift signal (its dominant frequency) when the run
-is long enough to show shedding, and the mass check: the mean density
+number St = D / (U T) of the lift signal with its standard error (T from the
+interpolated zero crossings of the second half, see strouhal()) when that
+half holds at least three shedding periods, and the mass check: the mean density
 deviation per node (sum of f over fluid nodes, FP64, imexlb_mass) and its
 difference from the FP64 run.
 """
@@ -49,26 +51,48 @@ def run_form(deck, prec, steps, every, r_vec):
     return np.array(forces), u, t_run, mass
 
 
-def strouhal(lift, every, D, U):
-    """Dominant frequency of the lift signal (second half), as f D / U."""
+def upward_crossings(x, every):
+    """Times (in lattice steps, sample i at i*every) where x - mean(x) passes
+    upward through zero, by linear interpolation between the two samples
+    that bracket it."""
+    x = np.asarray(x, dtype=float)
+    x = x - x.mean()
+    i = np.nonzero((x[:-1] < 0.0) & (x[1:] >= 0.0))[0]
+    frac = -x[i] / (x[i + 1] - x[i])
+    return (i + frac) * every
+
+
+def strouhal(lift, every, D, U, min_periods=3):
+    """Shedding frequency of the lift signal's second half as St = D / (U T).
+
+    T is the mean of the periods between consecutive upward zero crossings
+    (crossings interpolated between samples, upward_crossings), so a signal
+    sampled every `every` steps resolves periods far finer than the sample
+    spacing.  Returns (St, St_err, periods) with St_err the standard error
+    of that mean carried to St (the scatter of the individual periods over
+    the square root of their number: cycle-to-cycle variation of the wake,
+    and for a strictly periodic signal the interpolation error only), or
+    (None, None, n) when the second half holds fewer than min_periods whole
+    periods."""
     x = np.asarray(lift, dtype=float)
     x = x[len(x) // 2:]
-    if len(x) < 16:
-        return None
-    x = x - x.mean()
-    spec = np.abs(np.fft.rfft(x * np.hanning(len(x))))
-    freqs = np.fft.rfftfreq(len(x), d=every)  # cycles per lattice step
-    k = int(np.argmax(spec[1:])) + 1
-    if spec[k] <= 1e-12 * (np.abs(x).max() + 1e-300):
-        return None
-    return float(freqs[k] * D / U)
+    if len(x) < 8 or not np.isfinite(x).all():
+        return None, None, 0
+    periods = np.diff(upward_crossings(x, every))
+    n = len(periods)
+    if n < min_periods:
+        return None, None, n
+    T = periods.mean()
+    sT = periods.std(ddof=1) / np.sqrt(n)
+    st = D / (U * T)
+    return float(st), float(st * sT / T), int(n)
 
 
-def study(name, steps, forms, seed=2021):
+def study(name, steps, forms, seed=2021, log_every=10):
     deck = get_deck(name)
     d = derive(deck)
     r_vec = np.random.default_rng(seed).random(100)
-    every = max(d.interv, 1)
+    every = max(min(d.interv, log_every), 1)
     D = 2.0 * d.radius
     out = {"deck": name, "steps": steps, "log_every": every, "forms": {}}
     ref = None
@@ -78,8 +102,9 @@ def study(name, steps, forms, seed=2021):
         half = F[len(F) // 2:]
         res = {"MLUPS": round(nodes(deck) * steps / t / 1e6, 1), "drag_final": float(F[-1, 0]),
                "density_deviation_per_node": mass / nodes(deck),
-               "drag_mean_2nd_half": float(half[:, 0].mean()), "lift_final": float(F[-1, 1]),
-               "strouhal": strouhal(F[:, 1], every, D, d.uu)}
+               "drag_mean_2nd_half": float(half[:, 0].mean()), "lift_final": float(F[-1, 1])}
+        st, st_err, periods = strouhal(F[:, 1], every, D, d.uu)
+        res.update({"strouhal": st, "strouhal_err": st_err, "shedding_periods": periods})
         if form == "f64":
             ref = (F, u, mass)
         elif ref is not None:
@@ -89,6 +114,10 @@ def study(name, steps, forms, seed=2021):
             res["drag_final_rel_err"] = float(abs(F[-1, 0] - F64[-1, 0]) / abs(F64[-1, 0]))
             m64 = F64[len(F64) // 2:, 0].mean()
             res["drag_mean_rel_err"] = float(abs(half[:, 0].mean() - m64) / abs(m64))
+            st64 = out["forms"]["f64"]["strouhal"]
+            if st is not None and st64 is not None:
+                res["strouhal_diff"] = st - st64
+                res["strouhal_diff_err"] = float(np.hypot(st_err, out["forms"]["f64"]["strouhal_err"]))
         out["forms"][form] = res
     return out
 
@@ -98,11 +127,12 @@ def main(argv=None):
     ap.add_argument("deck")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--forms", default="f64,f32,f16,bf16")
+    ap.add_argument("--log-every", type=int, default=10, help="steps between drag/lift samples (at most interv)")
     a = ap.parse_args(argv)
     steps = a.steps
     if steps is None:
         steps = derive(get_deck(a.deck)).max_step + 1
-    print(json.dumps(study(a.deck, steps, a.forms.split(","))), flush=True)
+    print(json.dumps(study(a.deck, steps, a.forms.split(","), log_every=a.log_every)), flush=True)
 
 
 if __name__ == "__main__":

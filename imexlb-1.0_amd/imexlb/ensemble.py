@@ -114,6 +114,18 @@ class Batch:
         check(lib().imexlb_batch_forces(self._h, F.ctypes.data_as(_DP)), "imexlb_batch_forces")
         return F.reshape(self.ncase, 3)[:, :2]
 
+    def monitor(self):
+        """Monitor of every case: (um, diverged) with um[i] the case's
+        maximum |(u_x, u_y)| (NaN for a case with a non-finite velocity)
+        and diverged the indices of those cases."""
+        um = np.zeros(self.ncase)
+        rc = lib().imexlb_batch_monitor(self._h, um.ctypes.data_as(_DP))
+        if rc not in (_lib.OK, _lib.E_NAN):
+            check(rc, "imexlb_batch_monitor")
+        diverged = np.nonzero(np.isnan(um))[0]
+        assert (rc == _lib.E_NAN) == (diverged.size > 0)
+        return um, diverged
+
     def download_fields(self, i):
         n = self.nodes[i]
         p = np.zeros(n)
@@ -178,11 +190,12 @@ def main(argv=None):
     b.synchronize()
     t_batch = time.perf_counter() - t0
     F = b.forces()
+    _, diverged = b.monitor()
     b.close()
     out = {"cases": a.cases, "nodes": nodes, "precision": a.precision, "tuned": best,
            "batch_steps": a.steps, "batch_s": round(t_batch, 3),
            "batch_MLUPS": round(nodes * a.steps / t_batch / 1e6, 1),
-           "nonfinite_cases": int((~np.isfinite(F).all(axis=1)).sum()),
+           "nonfinite_cases": int(diverged.size), "diverged": diverged.tolist(),
            "drag_range": [float(np.nanmin(F[:, 0])), float(np.nanmax(F[:, 0]))]}
     if a.no_sequential:
         print(json.dumps(out), flush=True)

@@ -46,6 +46,8 @@ typedef struct imexlb_ctx imexlb_ctx;
 #define IMEXLB_E_UB (-5)     /* the reference's result depends on a read past
                                 the end of its arrays (undefined there)       */
 #define IMEXLB_E_NODEV (-6)  /* compute call on a host-only context (device<0) */
+#define IMEXLB_E_NAN (-7)    /* Monitor found a non-finite velocity: the
+                                lattice has diverged                          */
 
 /* storage / arithmetic forms */
 #define IMEXLB_F64 0       /* FP64 storage and arithmetic in the reference's
@@ -161,7 +163,11 @@ int imexlb_set_fused(imexlb_ctx *ctx, int fused);
 int imexlb_advance(imexlb_ctx *ctx, int64_t nsteps);
 
 /* Monitor's local maximum (lbm.f90:205-217): max over fluid entries 1..F-1
- * of sqrt(ux^2+uy^2).  Synchronises. */
+ * of sqrt(ux^2+uy^2).  Synchronises.  If any of those entries has a
+ * non-finite ux^2+uy^2 (NaN or infinity: the lattice has diverged), *um is
+ * set to NaN and the call returns IMEXLB_E_NAN, with the number of such
+ * nodes in imexlb_last_error(); the reference's max() would instead print
+ * whatever its NaN handling gives (lbm.f90:214). */
 int imexlb_monitor(imexlb_ctx *ctx, double *um);
 
 /* Force on the body by momentum exchange, in lattice units, for the most
@@ -249,6 +255,12 @@ int imexlb_batch_tune(imexlb_batch *b, int trial_steps, int32_t best[3], double 
 int imexlb_batch_synchronize(imexlb_batch *b);
 /* Momentum-exchange force of every case, F[3*i..3*i+2]. */
 int imexlb_batch_forces(imexlb_batch *b, double *F);
+/* Monitor of every case (lbm.f90:205-217 per case, as imexlb_monitor):
+ * um[i] = max over case i's fluid entries 1..F-1 of sqrt(ux^2+uy^2).  A
+ * case with a non-finite velocity gets um[i] = NaN; if any case has one the
+ * call returns IMEXLB_E_NAN after filling every um[i] (imexlb_last_error
+ * names the number of diverged cases).  Synchronises. */
+int imexlb_batch_monitor(imexlb_batch *b, double *um);
 /* p and u of case i, interior layout (u[d][y][x]). */
 int imexlb_batch_download_fields(imexlb_batch *b, int i, double *p, double *u);
 /* [0] nodes of all cases, [1] cases, [2] tiles, [3] device bytes */

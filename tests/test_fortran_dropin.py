@@ -163,3 +163,25 @@ def test_driver_storage_forms(prec, rtol, r_vec, tmp_path):
     assert len(printed) == len(um) > 3
     assert np.allclose(printed[1:], um[1:], rtol=rtol, atol=0), np.abs(printed / um - 1).max()
     o.close()
+
+
+@pytest.mark.gpu
+def test_driver_aborts_on_divergence(tmp_path):
+    """A deck far past BGK's stability limit (inlet velocity about 70
+    lattice units at ome 1.99) turns non-finite within ten steps: the first
+    Monitor after that (iteration 17) reports IMEXLB_E_NAN and the drop-in's
+    Monitor prints the error and calls MPI_ABORT, instead of printing a
+    maximum that left the NaN out."""
+    text = "48 24\n0.25 0.25 1000000\n1.99 1 1\n0.002 0.0001\n"
+    d = I.derive(I.parse_deck(text, 2))
+    assert d.interv == 17 and d.uu > 50
+    exe = os.path.join(FORTRAN, "build", "imexlb2d")
+    with open(os.path.join(str(tmp_path), "input.in"), "w") as fh:
+        fh.write(text)
+    print(f"[fortran] divergence deck in {tmp_path}", flush=True)
+    out = subprocess.run([exe], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    text_out = out.stdout + out.stderr
+    assert out.returncode != 0, text_out
+    assert "Monitor failed" in text_out and "non-finite" in text_out, text_out
+    assert len(re.findall(r"Umax=", text_out)) == 1  # only iteration 0 was printed
+    assert not os.path.exists(os.path.join(str(tmp_path), "output_001.plt"))

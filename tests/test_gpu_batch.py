@@ -323,3 +323,28 @@ def test_device_start_and_check_errors(r_vec):
         lat3.init_fields_f32(torch.zeros((3, 9, 17), device="cuda"), 1.0, 1.0)
     assert e.value.code == -1
     lat3.close()
+
+
+@pytest.mark.gpu
+def test_batch_monitor_flags_diverged_cases(r_vec):
+    """imexlb_batch_monitor: per-case Umax equal to the single lattices'
+    Monitor for stable cases, NaN and IMEXLB_E_NAN (reported, not raised, by
+    Batch.monitor) for a case run far past BGK's stability limit."""
+    decks = _small_cases()[:3]
+    bad = decks[1]
+    # the same grid at Re 1e6 and ome 1.99: inlet velocity far above 1
+    decks[1] = I.Deck(2, bad.n, bad.Lc, bad.Xpos, 1.0e6, 1.99, 1.0, 1.0, 1.0, 0.01)
+    assert I.derive(decks[1]).uu > 10
+    b = Batch(decks, precision=I.F64)
+    b.init(r_vec)
+    um0, div0 = b.monitor()
+    assert div0.size == 0 and np.isfinite(um0).all()
+    b.advance(30)
+    um, diverged = b.monitor()
+    assert diverged.tolist() == [1] and np.isnan(um[1])
+    for i in (0, 2):
+        lat, _ = I.setup(decks[i], r_vec, device=0)
+        lat.advance(30)
+        assert lat.monitor() == um[i], i
+        lat.close()
+    b.close()

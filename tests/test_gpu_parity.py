@@ -302,3 +302,71 @@ def test_precision_forms_error(precision, tol, r_vec):
     assert np.isfinite(err) and err < tol
     ref.close()
     lat.close()
+
+
+@pytest.mark.parametrize("fail_at", range(11, 21))
+def test_failed_set_geometry_leaves_no_geometry(fail_at, r_vec, monkeypatch):
+    """A SetGeometry whose device work fails part-way (IMEXLB_FAIL_ALLOC
+    makes the context's fail_at-th allocation fail; a 2D context makes 10
+    at creation, SetGeometry the next 10) leaves the context without a
+    geometry: InitUP, advance, Monitor, the list queries and the downloads
+    return IMEXLB_E_STATE instead of running on half-built tables.  A second
+    SetGeometry then succeeds and the run equals the oracle bit for bit."""
+    deck = get_deck("tiny2d")
+    d = I.derive(deck)
+    print(f"[parity] injected allocation failure at {fail_at}", flush=True)
+    monkeypatch.setenv("IMEXLB_FAIL_ALLOC", str(fail_at))
+    lat = I.Lattice(2, *deck.n[:2], device=0)
+    monkeypatch.delenv("IMEXLB_FAIL_ALLOC")
+    lat.set_params(deck.ome, deck.rho0, d.uu)
+    with pytest.raises(I.ImexlbError) as err:
+        lat.set_geometry(d.pos, d.radius)
+    assert err.value.code == I.E_OOM
+    for call in (lambda: lat.init_up(r_vec), lambda: lat.init_pdf(), lambda: lat.advance(2), lat.monitor,
+                 lambda: lat.geometry_list("b_user"), lat.download_fields, lat.forces, lat.mass):
+        with pytest.raises(I.ImexlbError) as err:
+            call()
+        assert err.value.code == I.E_STATE
+    lat.set_geometry(d.pos, d.radius)  # later allocations succeed
+    lat.init_up(r_vec)
+    lat.init_pdf()
+    lat.advance(20)
+    o, _ = oracle_setup(deck, r_vec)
+    o.run(0, 19, 0)
+    assert lat.monitor() == o.monitor()
+    assert_state_equal(lat, o)
+    lat.close()
+    o.close()
+
+
+def test_monitor_reports_nonfinite(r_vec):
+    """A NaN or an infinity in u at a fluid node makes Monitor return
+    IMEXLB_E_NAN (a diverged lattice is not reported as Umax = 0); one at
+    the first fluid node, which the reference's Monitor skips
+    (lbm.f90:212), is not seen; after InitPDF and a few steps the NaN has
+    spread and Monitor reports it again."""
+    deck = get_deck("tiny2d")
+    lat, d = I.setup(deck, r_vec, device=0)
+    lat.advance(5)
+    p, u = lat.download_fields(I.LAYOUT_REF)
+    fid = lat.geometry_list("fluid_id")
+    base = lat.monitor()
+    for bad in (np.nan, np.inf, -np.inf):
+        u2 = u.copy()
+        u2[2 * fid[len(fid) // 2] + 1] = bad  # u_y of a node mid-list
+        lat.upload_fields(p, u2, I.LAYOUT_REF)
+        with pytest.raises(I.ImexlbError) as err:
+            lat.monitor()
+        assert err.value.code == I.E_NAN and "1 fluid node" in str(err.value)
+    u2 = u.copy()
+    u2[2 * fid[0]] = np.nan  # the entry Monitor leaves out
+    lat.upload_fields(p, u2, I.LAYOUT_REF)
+    assert lat.monitor() == base
+    u2[2 * fid[len(fid) // 2]] = np.nan
+    lat.upload_fields(p, u2, I.LAYOUT_REF)
+    lat.init_pdf()
+    lat.advance(4)
+    with pytest.raises(I.ImexlbError) as err:
+        lat.monitor()
+    assert err.value.code == I.E_NAN
+    lat.close()

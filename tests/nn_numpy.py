@@ -7,14 +7,23 @@ by finite differences and its bf16 rounding by torch's conversion
 import numpy as np
 
 
+def _tap(xp, ky, kx, H, W):
+    """The input window of tap (ky, kx) as [N][CI][H*W] (a copy)."""
+    N, CI = xp.shape[:2]
+    return np.ascontiguousarray(xp[:, :, ky:ky + H, kx:kx + W]).reshape(N, CI, H * W)
+
+
 def conv3x3(x, w, b, relu):
-    """x [N][CI][H][W], w [CO][CI][3][3], b [CO]; padding 1."""
+    """x [N][CI][H][W], w [CO][CI][3][3], b [CO]; padding 1.  Each tap is a
+    matrix product w[:, :, ky, kx] @ window (BLAS)."""
     N, CI, H, W = x.shape
+    CO = w.shape[0]
     xp = np.pad(x, ((0, 0), (0, 0), (1, 1), (1, 1)))
-    out = np.zeros((N, w.shape[0], H, W))
+    out = np.zeros((N, CO, H * W))
     for ky in range(3):
         for kx in range(3):
-            out += np.einsum("oc,nchw->nohw", w[:, :, ky, kx], xp[:, :, ky:ky + H, kx:kx + W])
+            out += np.matmul(np.ascontiguousarray(w[:, :, ky, kx]), _tap(xp, ky, kx, H, W))
+    out = out.reshape(N, CO, H, W)
     if b is not None:
         out += b[None, :, None, None]
     return np.maximum(out, 0.0) if relu else out
@@ -23,14 +32,19 @@ def conv3x3(x, w, b, relu):
 def conv3x3_backward(x, w, out, dout, relu):
     """Gradients of conv3x3 (out is its output): (dx, dw, db)."""
     N, CI, H, W = x.shape
+    CO = w.shape[0]
     dz = dout * (out > 0) if relu else dout
     xp = np.pad(x, ((0, 0), (0, 0), (1, 1), (1, 1)))
     dw = np.zeros_like(w)
     dxp = np.zeros_like(xp)
+    dzm = np.ascontiguousarray(dz).reshape(N, CO, H * W)
+    dz2 = dzm.transpose(1, 0, 2).reshape(CO, N * H * W)
     for ky in range(3):
         for kx in range(3):
-            dw[:, :, ky, kx] = np.einsum("nohw,nchw->oc", dz, xp[:, :, ky:ky + H, kx:kx + W])
-            dxp[:, :, ky:ky + H, kx:kx + W] += np.einsum("oc,nohw->nchw", w[:, :, ky, kx], dz)
+            xt = _tap(xp, ky, kx, H, W)
+            dw[:, :, ky, kx] = dz2 @ np.ascontiguousarray(xt.transpose(1, 0, 2)).reshape(CI, N * H * W).T
+            wt = np.ascontiguousarray(w[:, :, ky, kx].T)
+            dxp[:, :, ky:ky + H, kx:kx + W] += np.matmul(wt, dzm).reshape(N, CI, H, W)
     return dxp[:, :, 1:-1, 1:-1], dw, dz.sum(axis=(0, 2, 3))
 
 

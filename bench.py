@@ -9,7 +9,16 @@ the timed region.  The CPU baseline is the oracle (the C restatement of the
 reference) run on the host cores over a few steps of the same lattice.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-variants]
+                    [--warm-start full|off] [--warm-start-args "..."]
     python bench.py --copy-calibration
+
+The metric's second half, wall time to drag within 1 % with the learned
+warm start against the plain (cold) start, comes from the warm-start leg
+(``warm_start`` in the JSON line): the whole pipeline of
+imexlb.surrogate on the 512 x 128 channel -- data from the batched solver,
+training of the 4-level network by the library's kernels, BF16 inference,
+then cold, warm and mean-field starts of the held-out cases -- with its
+wall-time ratios (inference and start included) and every phase's time.
 
 ``roofline.traffic`` (HBM bytes per launch of the step kernel) is read from
 the counter summary in profiles/: FETCH_SIZE and WRITE_SIZE of the step
@@ -74,6 +83,41 @@ def cpu_leg(deck, r_vec, steps):
     return t1 - t0
 
 
+def warm_start_leg(extra):
+    """The learned warm-start pipeline (imexlb.surrogate) at the sizes its
+    options give; the wall-time ratios warm / cold to drag within 1 % of the
+    converged value (median over the held-out cases that converge in both
+    and match the cold run's drag and lift), with the network's inference
+    and the starts included."""
+    import imexlb.surrogate as S
+    try:
+        a = S.build_parser().parse_args(extra)
+        r = S.pipeline(a, log=log)
+    except Exception as exc:  # the MLUPS legs stand; the line records the failure
+        return {"error": f"{type(exc).__name__}: {exc}"}
+    return {
+        "wall_ratio": r["median_wall_ratio_warm_over_cold"],
+        "wall_ratio_single_case": r["median_wall_ratio_single_case"],
+        "steps_ratio": r["median_ratio_warm_over_cold"],
+        "steps_ratio_mean_field": r["median_ratio_meanfield_over_cold"],
+        "target": "wall_ratio <= 1/3",
+        "frac_cases_wall_within_one_third": r["frac_wall_within_one_third"],
+        "median_steps": {"cold": r["median_steps_cold"], "warm": r["median_steps_warm"]},
+        "cases": {"train": r["train_cases"], "test": r["test_cases"], "scored": r["cases_scored"],
+                  "drag_match": r["drag_match"], "lift_match": r["lift_match"]},
+        "heldout_rel_rms": {"network_bf16": r["heldout_rel_rms"], "mean_field": r["mean_field_rel_rms"]},
+        "re_range": r["re_range"],
+        "re_note": "Re capped at 300 (the plan names 40-400): BGK diverges on this grid above u D / nu of about 16",
+        "config": {"workload": "D2Q9 channel 512x128, cylinders and ellipses (axis ratio 0.5-2, 0-90 deg), "
+                               "FP16 storage", "network": f"4-level encoder-decoder {r['chans']} channels, "
+                               f"{r['params']} parameters, FP32 training, BF16 inference",
+                   "epochs": r["epochs"], "eval_steps": r["eval_steps"], "seed": r["seed"]},
+        "seconds": {"data": r["data_s"], "train": r["train_s"], "pipeline": r["pipeline_s"],
+                    "start_batch_cold": r["start_seconds"]["batch_cold"],
+                    "start_batch_warm": r["start_seconds"]["batch_warm"]},
+    }
+
+
 def traffic_per_launch():
     """HBM bytes per step-kernel launch from the committed counter summary
     (its ``traffic_bytes_per_launch = N`` line), or None."""
@@ -109,6 +153,10 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--warm-start", choices=("full", "off"), default="full",
+                    help="run the learned warm-start pipeline (full: its default sizes)")
+    ap.add_argument("--warm-start-args", default="",
+                    help="options passed to imexlb.surrogate (e.g. '--train 1000 --epochs 20')")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -152,6 +200,11 @@ def main():
                "kind": "port",
                "sample": f"{args.cpu_steps} full steps of the 513x257x257 sphere lattice (oracle, OpenMP)"}
 
+    warm = None
+    if args.warm_start != "off":
+        log(f"[bench] learned warm start: imexlb.surrogate {args.warm_start_args}")
+        warm = warm_start_leg(args.warm_start_args.split())
+
     out = {
         "metric": "MLUPS (D3Q27 sphere, FP64) & % HBM roofline",
         "value": round(mlups, 1),
@@ -175,6 +228,7 @@ def main():
                    "launches": g["fused_launches"], "algorithmic_bytes_per_launch": bytes_per_launch,
                    "fill_avg_ms": round(g["fill_ms"] / max(g["fill_launches"], 1), 4)},
         "variants": variants,
+        "warm_start": warm,
     }
     print(json.dumps(out), flush=True)
 

@@ -2,6 +2,7 @@
 
     python -m imexlb.perf [--decks cyl400x100,sphere512x256x256,cyl8192x2048]
                           [--forms f64,f32,f16_f32,bf16_f32] [--seconds 2.0]
+                          [--no-graphs]
 
 For each (workload, form): set up the lattice on device 0, warm up, time
 ``--timed-steps`` steps launch by launch with HIP events (the average
@@ -13,11 +14,16 @@ and the algorithmic HBM rate of that kernel,
 MI355X peak of 8000 GB/s.  Lattices that fit in the 256 MB last-level cache
 (the 400x100 cylinder) are launch-bound, not HBM-bound; their GB/s is
 reported for completeness.
+
+--no-graphs launches every step on its own (IMEXLB_GRAPHS=0) instead of
+replaying captured step pairs: the form used under a profiler's kernel
+trace, where every launch then appears as its own dispatch.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import os
 import sys
 import time
 
@@ -75,7 +81,10 @@ def main(argv=None):
     ap.add_argument("--forms", default="f64,f32,f16_f32,bf16_f32")
     ap.add_argument("--seconds", type=float, default=2.0)
     ap.add_argument("--timed-steps", type=int, default=200)
+    ap.add_argument("--no-graphs", action="store_true", help="launch every step on its own (IMEXLB_GRAPHS=0)")
     a = ap.parse_args(argv)
+    if a.no_graphs:
+        os.environ["IMEXLB_GRAPHS"] = "0"  # read by every lattice context at creation
     for name in a.decks.split(","):
         for form in a.forms.split(","):
             print(f"[perf] {name} {form}", file=sys.stderr, flush=True)

@@ -535,7 +535,7 @@ def bench_inference(chans, sizes=(40, 8, 1), reps=10):
     print(json.dumps(out), flush=True)
 
 
-def main(argv=None):
+def build_parser():
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=2000)
     ap.add_argument("--test", type=int, default=200)
@@ -551,11 +551,15 @@ def main(argv=None):
                     help="multiply the predicted u_x by InitUP's perturbation (initup) or start from it as is")
     ap.add_argument("--inference-only", action="store_true",
                     help="time FP32 and BF16 inference of the network (random weights) and stop")
-    a = ap.parse_args(argv)
-    if a.inference_only:
-        return bench_inference([int(c) for c in a.chans.split(",")])
+    return ap
+
+
+def pipeline(a, log=lambda m: print(m, file=sys.stderr, flush=True)):
+    """Data, training, inference and the cold / warm / mean-field
+    evaluation for the options `a` (build_parser()); returns the result
+    record (a dict) with every phase's wall time."""
     import torch
-    log = lambda m: print(m, file=sys.stderr, flush=True)
+    t_all = time.perf_counter()
     prec = _lib.F16_F32
     r_vec = np.random.default_rng(a.seed).random(100)
     warm_rv = r_vec if a.warm_perturbation == "initup" else None
@@ -743,8 +747,17 @@ def main(argv=None):
                            "infer_fp32_ms": round(t_infer_one32 * 1e3, 3)},
            "batch_seconds": {"cold": round(sc, 2), "warm": round(sw, 2), "mean_field": round(sm, 2)},
            "start_seconds": {"batch_cold": round(ic, 4), "batch_warm": round(iw, 4),
-                             "single_cold": round(starts["cold"], 5), "single_warm": round(starts["warm"], 5)}}
-    print(json.dumps(out), flush=True)
+                             "single_cold": round(starts["cold"], 5), "single_warm": round(starts["warm"], 5)},
+           "re_range": list(RE_RANGE), "grid": [NX, NY], "eval_steps": a.eval_steps,
+           "pipeline_s": round(time.perf_counter() - t_all, 1)}
+    return out
+
+
+def main(argv=None):
+    a = build_parser().parse_args(argv)
+    if a.inference_only:
+        return bench_inference([int(c) for c in a.chans.split(",")])
+    print(json.dumps(pipeline(a)), flush=True)
 
 
 if __name__ == "__main__":

@@ -316,7 +316,7 @@ def test_full_size_network_matches_numpy():
     G = rng.normal(0, 1.0, ref.shape).astype(np.float32)
     net.backward(torch.tensor(G, device="cuda"))
     grads = R.backward(G.astype(np.float64), params, chans, cache)
-    assert len(grads) == len(net.dw) == 19
+    assert len(grads) == len(net.dw) == 15  # 4 encoder pairs, 3 decoder pairs, the output
     for k, (dw, db) in enumerate(grads):
         gw, gb = net.dw[k].cpu().numpy(), net.db[k].cpu().numpy()
         assert np.abs(gw - dw).max() <= 1e-3 * max(np.abs(dw).max(), 1e-6), k

@@ -118,6 +118,20 @@ def warm_start_leg(extra):
     }
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo) and its logical CPU count."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{name} ({os.cpu_count()} logical CPUs visible)"
+
+
 def traffic_per_launch():
     """HBM bytes per step-kernel launch from the committed counter summary
     (its ``traffic_bytes_per_launch = N`` line), or None."""
@@ -198,7 +212,8 @@ def main():
         cs = cpu_leg(deck, r_vec, args.cpu_steps)
         cpu = {"value": round(nodes * args.cpu_steps / cs / 1e6, 2), "unit": "MLUPS", "cores": threads,
                "kind": "port",
-               "sample": f"{args.cpu_steps} full steps of the 513x257x257 sphere lattice (oracle, OpenMP)"}
+               "sample": f"{args.cpu_steps} full steps of the 513x257x257 sphere lattice (oracle, OpenMP)",
+               "cpu_model": cpu_model()}
 
     warm = None
     if args.warm_start != "off":

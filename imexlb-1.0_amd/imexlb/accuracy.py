@@ -51,15 +51,26 @@ def run_form(deck, prec, steps, every, r_vec):
     return np.array(forces), u, t_run, mass
 
 
-def upward_crossings(x, every):
+def upward_crossings(x, every, hysteresis=0.25):
     """Times (in lattice steps, sample i at i*every) where x - mean(x) passes
     upward through zero, by linear interpolation between the two samples
-    that bracket it."""
+    that bracket it.  A crossing counts once the signal has come from below
+    -h and goes on to reach +h (h = hysteresis times its standard
+    deviation), so noise around zero does not count as extra cycles; its
+    time is the last upward zero crossing on that way up."""
     x = np.asarray(x, dtype=float)
     x = x - x.mean()
-    i = np.nonzero((x[:-1] < 0.0) & (x[1:] >= 0.0))[0]
-    frac = -x[i] / (x[i + 1] - x[i])
-    return (i + frac) * every
+    h = hysteresis * x.std()
+    out, low, last = [], False, None
+    for i in range(len(x) - 1):
+        if x[i] < -h:
+            low, last = True, None
+        if x[i] < 0.0 <= x[i + 1]:
+            last = i + (-x[i]) / (x[i + 1] - x[i])
+        if low and last is not None and x[i + 1] > h:
+            out.append(last * every)
+            low, last = False, None
+    return np.array(out)
 
 
 def strouhal(lift, every, D, U, min_periods=3):

@@ -103,11 +103,16 @@ def forward(x, params, chans):
     return out, (acts, enc)
 
 
-def backward(dout, params, chans, cache):
-    """Gradients [(dw, db)] in layer order for the loss gradient dout."""
+def backward(dout, params, chans, cache, abs_sums=None):
+    """Gradients [(dw, db)] in layer order for the loss gradient dout.  With
+    a list as abs_sums, it is filled with the sums of the magnitudes of the
+    terms of every gradient element [(sum |dz| |in|, sum |dz|)], the scale
+    of its rounding error in any summation order."""
     acts, enc = cache
     L = len(chans)
     grads = [None] * len(params)
+    if abs_sums is not None:
+        abs_sums[:] = [None] * len(params)
     k = len(params) - 1
 
     def conv_b(g):
@@ -115,6 +120,10 @@ def backward(dout, params, chans, cache):
         inp, o, relu = acts[k]
         dx, dw, db = conv3x3_backward(inp, params[k][0], o, g, relu)
         grads[k] = (dw, db)
+        if abs_sums is not None:
+            dz = np.abs(g * (o > 0) if relu else g)
+            _, dwa, dba = conv3x3_backward(np.abs(inp), params[k][0], None, dz, False)
+            abs_sums[k] = (dwa, dba)
         k -= 1
         return dx
 

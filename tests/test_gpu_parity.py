@@ -286,7 +286,9 @@ def test_mass_matches_oracle(name, steps, fused, r_vec):
     o.close()
 
 
-@pytest.mark.parametrize("precision,tol", [(I.F32, 1e-4), (I.F16_F32, 5e-2), (I.BF16_F32, 3e-1)])
+# measured on MI355X (relative L2 error of u after 2,000 steps): FP32 1.2e-6,
+# FP16 storage 6.1e-4, BF16 storage 1.2e-2; the bounds are three times those
+@pytest.mark.parametrize("precision,tol", [(I.F32, 3.6e-6), (I.F16_F32, 1.8e-3), (I.BF16_F32, 3.7e-2)])
 def test_precision_forms_error(precision, tol, r_vec):
     """FP32 arithmetic with FP32 / FP16 / BF16 storage against the FP64 run of
     the 400x100 cylinder after 2000 steps: relative L2 error of u below tol."""

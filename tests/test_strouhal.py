@@ -86,3 +86,19 @@ def test_too_short_or_diverged_gives_none():
     x = _signal(500.0, 20000, 10)
     x[-5] = np.nan
     assert strouhal(x, 10, D, U) == (None, None, 0)
+
+
+def test_noise_near_zero_adds_no_cycles():
+    """Rounding-like noise of a tenth of the amplitude on every sample makes
+    plain sign changes many times per period; the hysteresis keeps one
+    crossing per period and the estimate within 0.2 %."""
+    period = 2500.0
+    st_true = D / (U * period)
+    x = _signal(period, 60000, 10, noise=0.1, seed=3)
+    xm = x - x.mean()
+    plain = np.count_nonzero((xm[:-1] < 0) & (xm[1:] >= 0))
+    assert plain > 3 * 60000 / period  # the noise alone crosses zero far more often
+    t = upward_crossings(x, 10)
+    assert abs(len(t) - 60000 / period) <= 1
+    st, err, n = strouhal(x, 10, D, U)
+    assert abs(st - st_true) <= 2e-3 * st_true and abs(st - st_true) <= 4 * err

@@ -28,8 +28,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
+#include <string>
 
 #include "../../include/imexlb_nn.h"
+#include "context.hpp"
 
 namespace imexlb_nn {
 
@@ -500,6 +503,72 @@ int imexlb_nn_conv3x3_dw_ordered(const float *in, const float *dout, float *dw, 
     const int64_t tiles = int64_t((W + TS - 1) / TS) * ((H + TS - 1) / TS);
     const int64_t n = int64_t(CO) * CI * 9;
     if (work_floats < tiles * n) return -1;
+    const dim3 grid(unsigned(tiles), unsigned(((CO + WCO - 1) / WCO) * ((CI + WCI - 1) / WCI)));
+    hipLaunchKernelGGL(k_conv3_dw, grid, dim3(256), 0, (hipStream_t)stream, in, dout, dw, N, CI, CO, H, W, work);
+    if (hipGetLastError() != hipSuccess) return -2;
+    NN_LAUNCH(k_dw_reduce, dim3(blocks(n)), dim3(256), 0, (hipStream_t)stream, work, dw, tiles, n);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The ordered weight gradient without a caller workspace: the per-tile sums
+// go to `scratch`, a device array of the library that every GPU it runs on
+// holds until the process ends.  Host code reaches it only through the
+// address hipGetSymbolAddress gives on each device (the array's name is a
+// host-side stand-in).  Nothing in it needs to start at zero: k_conv3_dw
+// writes every partial that k_dw_reduce then reads.  Calls that overlap on
+// one GPU share the array and race.
+constexpr int64_t kScratchFloats = int64_t(8) << 20;  // 8 Mi floats, 32 MiB
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+float *scratch_dev[kMaxDevices] = {};
+
+// this device's copy of scratch (the caller's current device), looked up on
+// the first call on the device; nullptr with the error text set otherwise
+float *scratch_here()
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        imexlb::fail(IMEXLB_E_HIP, std::string("no usable current device: ") + hipGetErrorString(e));
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            imexlb::fail(IMEXLB_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        scratch_dev[dev] = static_cast<float *>(p);
+    }
+    return scratch_dev[dev];
+}
+
+}  // namespace
+
+extern "C" {
+
+int imexlb_nn_conv3x3_dw_ordered_noalloc(const float *in, const float *dout, float *dw, int N, int CI, int CO,
+                                         int H, int W, void *stream)
+{
+    if (!in || !dout || !dw || N <= 0 || CI <= 0 || CO <= 0 || H <= 0 || W <= 0) return -1;
+    const int64_t tiles = int64_t((W + TS - 1) / TS) * ((H + TS - 1) / TS);
+    const int64_t n = int64_t(CO) * CI * 9;
+    if (tiles * n > kScratchFloats) {
+        imexlb::fail(IMEXLB_E_ARG, "weight gradient needs " + std::to_string(tiles * n) +
+                                       " floats of partial sums; the library's scratch holds " +
+                                       std::to_string(kScratchFloats));
+        return -1;
+    }
+    float *work = scratch_here();
+    if (!work) return -2;
     const dim3 grid(unsigned(tiles), unsigned(((CO + WCO - 1) / WCO) * ((CI + WCI - 1) / WCI)));
     hipLaunchKernelGGL(k_conv3_dw, grid, dim3(256), 0, (hipStream_t)stream, in, dout, dw, N, CI, CO, H, W, work);
     if (hipGetLastError() != hipSuccess) return -2;

@@ -79,7 +79,9 @@ SIGNATURES = {
     "imexlb_nn_conv3x3_dw_work": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "imexlb_nn_conv3x3_dw_ordered": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                                _P]),
-    "imexlb_nn_bias_grad": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
+    "imexlb_nn_conv3x3_dw_ordered_noalloc": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                       _P]),
+    "imexlb_nn_bias_grad":(C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
     "imexlb_nn_relu_bwd": (C.c_int, [_P, _P, _P, C.c_int64, _P]),
     "imexlb_nn_pool2": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
     "imexlb_nn_pool2_bwd": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),

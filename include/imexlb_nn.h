@@ -30,6 +30,19 @@ int imexlb_nn_conv3x3_dw(const float *in, const float *dout, float *dw, int N, i
 int64_t imexlb_nn_conv3x3_dw_work(int CI, int CO, int H, int W);
 int imexlb_nn_conv3x3_dw_ordered(const float *in, const float *dout, float *dw, float *work, int64_t work_floats,
                                  int N, int CI, int CO, int H, int W, void *stream);
+/* imexlb_nn_conv3x3_dw_ordered without the workspace argument, for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * training loops, harnesses that forbid hipMalloc); no counterpart in the
+ * reference.  It allocates nothing: the per-tile sums go to a device array
+ * of the library, 8,388,608 floats (32 MiB) on every GPU the library runs
+ * on, held until the process ends.  Shapes whose
+ * imexlb_nn_conv3x3_dw_work(CI, CO, H, W) exceeds that are rejected (-1;
+ * imexlb_last_error says why); every layer of the 4-level 32..256-channel
+ * network on 128 x 512 fits.  Results equal imexlb_nn_conv3x3_dw_ordered's
+ * bit for bit.  Runs on the current device; calls that overlap on one GPU
+ * (several streams or threads) share the array and race: serialise them. */
+int imexlb_nn_conv3x3_dw_ordered_noalloc(const float *in, const float *dout, float *dw, int N, int CI, int CO,
+                                         int H, int W, void *stream);
 /* db[c] = sum of dout[:, c] */
 int imexlb_nn_bias_grad(const float *dout, float *db, int N, int C, int H, int W, void *stream);
 /* dz = dout where out > 0, else 0 (out: the ReLU output) */

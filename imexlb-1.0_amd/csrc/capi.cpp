@@ -316,6 +316,7 @@ int imexlb_create(imexlb_ctx **out, int dim, int nx, int ny, int nz, int device,
         c->graphs = !(gv && gv[0] == '0');
         const char *pv = std::getenv("IMEXLB_PUSH");
         c->push_enabled = !(pv && pv[0] == '0');
+        if (const char *sv = std::getenv("IMEXLB_SMALL_CELLS")) c->small_cells = std::atoll(sv);
         HIP_TRY(hipSetDevice(device));
         HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         const size_t pop = size_t(c->Q) * size_t(c->plane) * c->ssz;

@@ -208,8 +208,15 @@ struct imexlb_ctx {
         });
     }
 
-    // mode bits of the fused collide step for this lattice
-    int collide_mode() const { return push_on ? (K_COLLIDE | K_PUSH | (fill32 ? 0 : K_IDX64)) : K_COLLIDE; }
+    // mode bits of the fused collide step for this lattice (K_SMALL for 2D
+    // lattices of at most small_cells padded cells)
+    int collide_mode() const
+    {
+        if (!push_on) return K_COLLIDE;
+        if (!fill32) return K_COLLIDE | K_PUSH | K_IDX64;
+        return K_COLLIDE | K_PUSH | (dim == 2 && int64_t(kp.ncell) <= small_cells ? K_SMALL : 0);
+    }
+    int64_t small_cells = int64_t(1) << 18;  // IMEXLB_SMALL_CELLS
 
     int run_step(int mode, const void *src, void *dst, void *fout)
     {

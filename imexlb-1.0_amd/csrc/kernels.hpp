@@ -90,8 +90,13 @@ inline int64_t pop_slot(int64_t q, int64_t c, int64_t plane, int Q)
 
 // K_PUSH: the collide step also writes the boundary copies of its output
 // (PushTable), so the buffer needs no boundary fill; K_IDX64: the push
-// table's destination slots are int64 (else int32).
-enum : int { K_COLLIDE = 1, K_WRITE_F = 2, K_WRITE_PU = 4, K_NOSHIFT = 8, K_PUSH = 16, K_IDX64 = 32 };
+// table's destination slots are int64 (else int32); K_SMALL: a small,
+// latency-bound lattice -- every cell loads its push-table range together
+// with its populations instead of after its node type (4 more bytes per
+// cell), the output is written with ordinary (cached) stores, and blocks
+// have kSmallBlock threads (more of the CUs take part).
+enum : int { K_COLLIDE = 1, K_WRITE_F = 2, K_WRITE_PU = 4, K_NOSHIFT = 8, K_PUSH = 16, K_IDX64 = 32, K_SMALL = 64 };
+constexpr unsigned kSmallBlock = 128;
 
 // node-type byte on the device: fluid node; node with boundary copies to push
 enum : uint8_t { G_FLUID = 1, G_PUSH = 2 };

@@ -477,6 +477,44 @@ __device__ __forceinline__ void gather_wave(const S *__restrict__ src, const KPa
     }
 }
 
+// Entries [b, e) of the push table that cell c copies out: empty unless
+// K_PUSH and the cell has copies.  K_SMALL reads the range of every
+// cell, so the load does not wait for the node type.
+struct PushRange {
+    int32_t b, e;
+};
+template <int MODE>
+__device__ __forceinline__ PushRange push_range(const PushTable &pt, uint8_t g, uint32_t c)
+{
+    PushRange r{0, 0};
+    if ((MODE & K_PUSH) && ((MODE & K_SMALL) || (g & G_PUSH))) {
+        r.b = pt.beg[c];
+        r.e = pt.beg[c + 1];
+    }
+    return r;
+}
+
+// v[q] for a direction known only at run time, as an OR of masked bit
+// patterns: no indexed access, which would move v to scratch memory
+template <int Q, class S>
+__device__ __forceinline__ S pick(const S (&v)[Q], uint32_t q)
+{
+    using U = std::conditional_t<sizeof(S) == 8, uint64_t, std::conditional_t<sizeof(S) == 4, uint32_t, uint16_t>>;
+    U r = 0;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) r |= __builtin_bit_cast(U, v[k]) & U(U(0) - U(q == uint32_t(k)));
+    return __builtin_bit_cast(S, r);
+}
+
+// Boundary copies of D2Q9 nodes are written from registers (the node's
+// output values, or a body cell's own values loaded at once) with the first
+// IMEXLB_PUSH_PREFETCH table entries loaded before the node's arithmetic:
+// on small lattices the copies are the longest chain of dependent memory
+// accesses in the step.  D3Q27 keeps one read of the value per copy.
+#ifndef IMEXLB_PUSH_PREFETCH
+#define IMEXLB_PUSH_PREFETCH 4
+#endif
+
 // The rest of a node update from its gathered populations and node type g.
 template <int D, class S, class C, int MODE>
 __device__ __forceinline__ void finish_node(const C (&f)[Lat<D>::Q], uint8_t g, const S *__restrict__ src,
@@ -508,6 +546,21 @@ __device__ __forceinline__ void finish_node(const C (&f)[Lat<D>::Q], uint8_t g, 
 {
     using L = Lat<D>;
     const bool fluid = (g & G_FLUID) != 0;
+    // D2Q9 copies: the cell's range and its first entries, before any arithmetic
+    constexpr bool kRegPush = (MODE & K_PUSH) && D == 2;
+    constexpr int PD = kRegPush ? IMEXLB_PUSH_PREFETCH : 0;
+    PushRange pr{0, 0};
+    uint32_t pq[PD > 0 ? PD : 1];
+    uint64_t ps[PD > 0 ? PD : 1];
+    if constexpr (kRegPush) {
+        pr = push_range<MODE>(pt, g, c);
+#pragma unroll
+        for (int k = 0; k < PD; ++k)
+            if (pr.b + k < pr.e) {
+                pq[k] = pt.q[pr.b + k];
+                ps[k] = push_slot<MODE>(pt, pr.b + k);
+            }
+    }
     if (!fluid) {
         if (MODE & K_WRITE_PU) {  // PostProcessing's inlet/outlet loops also reach solid nodes
             if (x == 0) {
@@ -518,7 +571,17 @@ __device__ __forceinline__ void finish_node(const C (&f)[Lat<D>::Q], uint8_t g, 
             }
             if (x == kp.nx) pout[c] = 0.0;
         }
-        if ((MODE & K_PUSH) && (g & G_PUSH)) {  // a body cell: its previous-step values
+        if constexpr (kRegPush) {  // a body cell: its previous-step values
+            if (pr.b < pr.e) {
+                S own[L::Q];
+#pragma unroll
+                for (int q = 0; q < L::Q; ++q) own[q] = src[cslot<L::Q>(kp, q, c)];
+#pragma unroll
+                for (int k = 0; k < PD; ++k)
+                    if (pr.b + k < pr.e) dst[ps[k]] = pick(own, pq[k]);
+                for (int32_t e = pr.b + PD; e < pr.e; ++e) dst[push_slot<MODE>(pt, e)] = pick(own, pt.q[e]);
+            }
+        } else if ((MODE & K_PUSH) && (g & G_PUSH)) {
             const int32_t e1 = pt.beg[c + 1];
             for (int32_t e = pt.beg[c]; e < e1; ++e) dst[push_slot<MODE>(pt, e)] = src[cslot<L::Q>(kp, pt.q[e], c)];
         }
@@ -541,14 +604,24 @@ __device__ __forceinline__ void finish_node(const C (&f)[Lat<D>::Q], uint8_t g, 
         // K_WAVE: the wave is one tile; its output slots are a scalar tile
         // base plus the lane
         S *const wdst = (MODE & K_WAVE) ? dst + uint64_t(wave_tile(c)) * (L::Q * 64u) : dst;
+        S out[kRegPush ? L::Q : 1];  // the stored values, for the copies
         collide<D, C>(f, p, ux, uy, uz, kp, [&](int q, C v) {
             S *o = (MODE & K_WAVE) ? wdst + (uint32_t(store_q(q, L::Q)) * 64u + (c & 63u)) : dst + cslot<L::Q>(kp, q, c);
-            if (IMEXLB_STEP_NT)
-                nt_store(o, Store<S>::template st<C>(v));
+            const S sv = Store<S>::template st<C>(v);
+            if constexpr (kRegPush) out[q] = sv;
+            if (IMEXLB_STEP_NT && !(MODE & K_SMALL))
+                nt_store(o, sv);
             else
-                *o = Store<S>::template st<C>(v);
+                *o = sv;
         });
-        if ((MODE & K_PUSH) && (g & G_PUSH)) {  // boundary copies of this node's output (read back)
+        if constexpr (kRegPush) {  // boundary copies of this node's output
+            if (pr.b < pr.e) {
+#pragma unroll
+                for (int k = 0; k < PD; ++k)
+                    if (pr.b + k < pr.e) dst[ps[k]] = pick(out, pq[k]);
+                for (int32_t e = pr.b + PD; e < pr.e; ++e) dst[push_slot<MODE>(pt, e)] = pick(out, pt.q[e]);
+            }
+        } else if ((MODE & K_PUSH) && (g & G_PUSH)) {  // boundary copies of this node's output (read back)
             const int32_t e1 = pt.beg[c + 1];
             for (int32_t e = pt.beg[c]; e < e1; ++e) dst[push_slot<MODE>(pt, e)] = dst[cslot<L::Q>(kp, pt.q[e], c)];
         }
@@ -978,9 +1051,11 @@ static hipError_t step_dm(int prec, const StepArgs &a, hipStream_t s)
     return dispatch_prec(prec, [&](auto *sp, auto *cp) {
         using S = std::remove_pointer_t<decltype(sp)>;
         using C = std::remove_pointer_t<decltype(cp)>;
-        constexpr uint64_t per_block = uint64_t(IMEXLB_STEP_BLOCK) * step_cpt<MODE>();
+        constexpr unsigned block = (MODE & K_SMALL) ? kSmallBlock : IMEXLB_STEP_BLOCK;
+        static_assert(block <= IMEXLB_STEP_BLOCK, "block within the kernel's launch bounds");
+        constexpr uint64_t per_block = uint64_t(block) * step_cpt<MODE>();
         const unsigned nb = unsigned((node_threads(a.kp) + per_block - 1) / per_block);
-        hipLaunchKernelGGL((k_step<D, S, C, MODE>), dim3(nb), dim3(IMEXLB_STEP_BLOCK), 0, s, (const S *)a.src,
+        hipLaunchKernelGGL((k_step<D, S, C, MODE>), dim3(nb), dim3(block), 0, s, (const S *)a.src,
                            (S *)a.dst, (S *)a.fout, a.p, a.ux, a.uy, a.uz, a.geo, a.kp, a.push);
         return hipGetLastError();
     });
@@ -993,6 +1068,7 @@ static hipError_t step_d(int prec, int mode, const StepArgs &a, hipStream_t s)
     case K_COLLIDE: return step_dm<D, K_COLLIDE>(prec, a, s);
     case K_COLLIDE | K_PUSH: return step_dm<D, K_COLLIDE | K_PUSH>(prec, a, s);
     case K_COLLIDE | K_PUSH | K_IDX64: return step_dm<D, K_COLLIDE | K_PUSH | K_IDX64>(prec, a, s);
+    case K_COLLIDE | K_PUSH | K_SMALL: return step_dm<D, K_COLLIDE | K_PUSH | K_SMALL>(prec, a, s);
     case K_WRITE_F: return step_dm<D, K_WRITE_F>(prec, a, s);
     case K_WRITE_F | K_WRITE_PU: return step_dm<D, K_WRITE_F | K_WRITE_PU>(prec, a, s);
     case K_NOSHIFT | K_WRITE_PU: return step_dm<D, K_NOSHIFT | K_WRITE_PU>(prec, a, s);

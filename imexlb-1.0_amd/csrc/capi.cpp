@@ -303,7 +303,7 @@ int imexlb_create(imexlb_ctx **out, int dim, int nx, int ny, int nz, int device,
     // XCD-contiguous block order pays for D3Q27 FP64 only (measured on
     // MI355X); IMEXLB_XCD_REMAP=0/1 overrides
     const char *xv = std::getenv("IMEXLB_XCD_REMAP");
-    k.xcd_remap = xv ? uint32_t(xv[0] == '1') : uint32_t(dim == 3 && precision == IMEXLB_F64);
+    k.xcd_remap = xv ? uint32_t(std::max(0, std::atoi(xv))) : uint32_t(dim == 3 && precision == IMEXLB_F64);
 
     if (const char *fv = std::getenv("IMEXLB_FAIL_ALLOC")) c->fail_alloc_at = std::atoll(fv);
     if (device >= 0) {

@@ -57,7 +57,7 @@ struct KParams {
     FastDiv div_px;      // / Px
     FastDiv div_py;      // / Py
     double inv_rho0;     // 1/rho0 (FP32-compute forms multiply by it)
-    uint32_t xcd_remap;  // 1: contiguous runs of blocks per XCD
+    uint32_t xcd_remap;  // block order over the XCDs: 0 hardware, 1 an eighth each, R >= 2 runs of R
     float ome_f, rho0_f, uu_f, inv_rho0_f;  // the same scalars rounded to FP32 once
     uint32_t back_tiles; // ceil((1 + Px + PxPy) / 64): the farthest tile a pull reads behind its own
 };

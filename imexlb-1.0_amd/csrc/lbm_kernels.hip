@@ -142,16 +142,24 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &d)
     return uint32_t((uint64_t(__umulhi(n, d.m)) + n) >> d.s);
 }
 
-// Blocks are dealt round-robin to the 8 XCDs; with `remap` each XCD gets a
-// contiguous run of blocks, so the cache lines two neighbouring blocks share
-// stay in one L2 (faster for D3Q27 FP64, slower for the 16-bit forms and
-// D2Q9 on MI355X; the lattice context chooses).
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb, bool remap)
+// Blocks are dealt round-robin to the 8 XCDs, so two neighbouring blocks
+// run on different XCDs and each fetches the cache lines they share into
+// its own L2.  remap = 1 gives each XCD one contiguous eighth of the
+// blocks; remap = R >= 2 gives each XCD runs of R consecutive blocks, the
+// eight XCDs working on one window of 8R blocks at a time; 0 keeps the
+// hardware order.  The lattice context chooses (measured on MI355X).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb, uint32_t remap)
 {
-    if (!remap) return b;
-    const uint32_t per = nb / 8u;
-    if (b >= per * 8u) return b;
-    return (b % 8u) * per + b / 8u;
+    if (remap == 0) return b;
+    if (remap == 1) {
+        const uint32_t per = nb / 8u;
+        if (b >= per * 8u) return b;
+        return (b % 8u) * per + b / 8u;
+    }
+    const uint32_t span = 8u * remap;
+    if (b >= nb / span * span) return b;  // the last partial window in hardware order
+    const uint32_t r = b % span;          // XCD r % 8, position r / 8 of its run
+    return b - r + (r % 8u) * remap + r / 8u;
 }
 
 template <int D>
@@ -651,7 +659,7 @@ __global__ IMEXLB_STEP_BOUNDS void k_step(const S *__restrict__ src, S *__restri
                                               KParams kp, PushTable pt)
 {
     constexpr int CPT = step_cpt<MODE>();
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, kp.xcd_remap != 0);
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, kp.xcd_remap);
     if constexpr (CPT == 1) {
         int x, y, z;
         uint32_t c;
@@ -754,7 +762,7 @@ __global__ __launch_bounds__(256) void k_collide(const S *__restrict__ fin, S *_
     using L = Lat<D>;
     int x, y, z;
     uint32_t c;
-    const uint32_t i = xcd_block(blockIdx.x, gridDim.x, kp.xcd_remap != 0) * blockDim.x + threadIdx.x;
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x, kp.xcd_remap) * blockDim.x + threadIdx.x;
     if (!node_of<D>(kp, i, x, y, z, c)) return;
     if (!(geo[c] & G_FLUID)) return;
     C f[L::Q];
@@ -774,7 +782,7 @@ __global__ __launch_bounds__(256) void k_init_pdf(S *__restrict__ fout, const do
     using L = Lat<D>;
     int x, y, z;
     uint32_t c;
-    const uint32_t i = xcd_block(blockIdx.x, gridDim.x, kp.xcd_remap != 0) * blockDim.x + threadIdx.x;
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x, kp.xcd_remap) * blockDim.x + threadIdx.x;
     if (!node_of<D>(kp, i, x, y, z, c)) return;
     if (!(geo[c] & G_FLUID)) return;
     const C pp = C(p[c]), u0 = C(ux[c]), u1 = C(uy[c]), u2 = D == 3 ? C(uz[c]) : C(0);

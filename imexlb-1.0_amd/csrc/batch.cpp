@@ -282,7 +282,7 @@ int imexlb_batch_create_shapes(imexlb_batch **out, int ncase, const int32_t *nxn
     const size_t nblk = (size_t(std::max<int64_t>(b->max_links, 1)) + 255) / 256;
     imexlb_batch *B = b.get();
     int rc;
-    B->fill32 = fill_fits_32(B->pop_total);
+    B->fill32 = fill_fits_32(B->pop_total) && !force_slots64();
     const size_t isz = B->fill32 ? 4 : 8;
     if ((rc = balloc(B, &B->fb[0], pop)) || (rc = balloc(B, &B->fb[1], pop)) || (rc = balloc(B, &B->f, pop)) ||
         (rc = balloc(B, (void **)&B->p, fld)) || (rc = balloc(B, (void **)&B->ux, fld)) ||

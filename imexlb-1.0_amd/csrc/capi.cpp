@@ -427,7 +427,7 @@ int imexlb_set_geometry_shape(imexlb_ctx *c, double pos, double radius, double a
     c->nlink = int64_t(t.link_out.size());
     const size_t nf = std::max<size_t>(t.fill_dst.size(), 1), nl = std::max<size_t>(t.link_out.size(), 1);
     const size_t nblk = (nl + 255) / 256;
-    c->fill32 = fill_fits_32(int64_t(c->Q) * c->plane);
+    c->fill32 = fill_fits_32(int64_t(c->Q) * c->plane) && !force_slots64();
     const size_t isz = c->fill32 ? 4 : 8;
     if ((rc = dev_alloc(c, &c->fill_dst, nf * isz)) || (rc = dev_alloc(c, &c->fill_src, nf * isz)) ||
         (rc = dev_alloc(c, (void **)&c->link_out, nl * 8)) || (rc = dev_alloc(c, (void **)&c->link_back, nl * 8)) ||

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -314,6 +315,14 @@ int build_device_tables(const imexlb_ctx *c, DeviceTables &t);
 // The boundary fill's slot tables fit int32 entries when every slot and its
 // previous-step encoding (-slot-2) do.
 inline bool fill_fits_32(int64_t nslots) { return nslots < (int64_t(1) << 31) - 2; }
+
+// test hook: IMEXLB_SLOTS64=1 (read when the boundary tables are built)
+// stores them with int64 slots also where int32 would do
+inline bool force_slots64()
+{
+    const char *v = std::getenv("IMEXLB_SLOTS64");
+    return v && v[0] == '1';
+}
 
 // copy a slot table to the device as int32 or int64 entries
 inline hipError_t upload_slots(void *dev, const std::vector<int64_t> &v, bool idx32)

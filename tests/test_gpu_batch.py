@@ -52,6 +52,31 @@ def test_batch_matches_single_lattices(block, cpt, persist, r_vec):
 
 
 @pytest.mark.gpu
+def test_batch_64bit_slot_tables(r_vec, monkeypatch):
+    """The batch's boundary fill with int64 slot tables (IMEXLB_SLOTS64=1,
+    the form an ensemble above 2^31 population slots takes) gives the same
+    fields and forces as its int32 tables, bit for bit."""
+    decks = _small_cases()
+    runs = []
+    for s64 in ("0", "1"):
+        monkeypatch.setenv("IMEXLB_SLOTS64", s64)
+        b = Batch(decks, precision=I.F64)
+        b.set_config(128, 2, 0)
+        b.init(r_vec)
+        b.advance(41)
+        runs.append((b, b.forces()))
+    (b0, F0), (b1, F1) = runs
+    assert np.array_equal(F0, F1)
+    for i in range(len(decks)):
+        p0, u0 = b0.download_fields(i)
+        p1, u1 = b1.download_fields(i)
+        assert np.isfinite(p0).all()
+        assert np.array_equal(p0, p1) and np.array_equal(u0, u1), i
+    b0.close()
+    b1.close()
+
+
+@pytest.mark.gpu
 def test_batch_tune_and_continue(r_vec):
     """Tuning picks a valid shape; advancing after it (odd and even step
     counts, through the graph and the single-step path) stays exact."""

@@ -116,6 +116,26 @@ def test_cylinder_8192x2048_bitwise(r_vec):
     o.close()
 
 
+def test_sphere_above_2_31_slots_bitwise(r_vec):
+    """A D3Q27 sphere lattice of 640 x 352 x 352 (81.0 M padded cells, 2.19 G
+    population slots: past 2^31, so the boundary tables hold int64 slots and
+    the step takes its K_IDX64 form), FP64, the sphere deck's flow
+    parameters: 2 steps equal the oracle bit for bit (f, p, u, Monitor).
+    About 55 GB of device memory and 75 GB of host memory."""
+    deck = I.parse_deck("640 352 352\n0.078125 0.25 200\n1.9083969465648855 1 1\n0.024997 0.0005001\n", 3)
+    cells = (640 + 3) * (352 + 3) * (352 + 3)
+    assert 27 * cells > 2 ** 31
+    print("[configs] sphere 640x352x352 FP64 (int64 slots) vs oracle", flush=True)
+    lat, d = I.setup(deck, r_vec, device=0)
+    lat.advance(2)
+    o, _ = oracle_setup(deck, r_vec)
+    o.run(0, 1, 0)
+    assert lat.monitor() == o.monitor()
+    assert_state_equal(lat, o)
+    lat.close()
+    o.close()
+
+
 _FP64_RUNS = {}  # (workload, steps) -> the FP64 run every form is compared with
 
 

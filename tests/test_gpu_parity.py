@@ -124,6 +124,26 @@ def test_small_lattice_mode_does_not_change_results(small_cells, r_vec, monkeypa
         o.close()
 
 
+@pytest.mark.parametrize("push", ["1", "0"])
+def test_64bit_slot_tables_match_oracle(push, r_vec, monkeypatch):
+    """Lattices above 2^31 population slots keep their boundary tables in
+    int64 slots (the step's K_IDX64 form, the fill's int64 tables);
+    IMEXLB_SLOTS64=1 selects them for small decks too.  Fused with pushed
+    copies or with the separate fill, fused and per phase: equal to the
+    oracle."""
+    monkeypatch.setenv("IMEXLB_SLOTS64", "1")
+    monkeypatch.setenv("IMEXLB_PUSH", push)
+    for name, steps, fused in (("inletbody2d", 70, True), ("wallsphere3d", 36, True), ("tiny2d", 31, False),
+                               ("cyl400x100", 100, True)):
+        lat, o, d = _run_pair(name, r_vec, steps, fused)
+        lat.advance(steps)
+        o.run(0, steps - 1, 0)
+        assert lat.monitor() == o.monitor()
+        assert_state_equal(lat, o)
+        lat.close()
+        o.close()
+
+
 def test_separate_fill_path_matches_oracle(r_vec, monkeypatch):
     """With IMEXLB_PUSH=0 the boundary copies run as their own fill kernel
     before every pull (instead of being written by the previous step);

@@ -701,12 +701,13 @@ static int one_step(imexlb_ctx *c)
 // one HIP graph: per pair fill(1-X) pull(1-X -> X) fill(X) pull(X -> 1-X),
 // or without the fills when the pulls push their boundary copies.  After it
 // the state is again S_FUSED with cur == X, so the graph replays.
-// Only kernels are recorded (no memset or copy nodes).  pairs: 1, or
-// kLongPairs (fewer launches for small lattices).
-static int step_graph(imexlb_ctx *c, int pairs, hipGraphExec_t *out)
+// Only kernels are recorded (no memset or copy nodes).  size: index into
+// kGraphPairs (the graph's number of pairs).
+static int step_graph(imexlb_ctx *c, int size, hipGraphExec_t *out)
 {
     const int X = c->cur;
-    hipGraphExec_t &slot = pairs == 1 ? c->graph[X] : c->graph_long[X];
+    const int pairs = imexlb_ctx::kGraphPairs[size];
+    hipGraphExec_t &slot = c->graph[size][X];
     if (slot) {
         *out = slot;
         return IMEXLB_OK;
@@ -752,22 +753,15 @@ int imexlb_advance(imexlb_ctx *c, int64_t nsteps)
             int rc = one_step(c);
             if (rc) return rc;
         }
-        const int64_t longs = (nsteps - i) / (2 * imexlb_ctx::kLongPairs);
-        if (longs > 0) {
+        for (int j = 0; j < imexlb_ctx::kGraphSizes; ++j) {
+            const int per = imexlb_ctx::kGraphPairs[j];
+            const int64_t reps = (nsteps - i) / (2 * per);
+            if (reps <= 0) continue;
             hipGraphExec_t g;
-            int rc = step_graph(c, imexlb_ctx::kLongPairs, &g);
+            int rc = step_graph(c, j, &g);
             if (rc) return rc;
-            for (int64_t k = 0; k < longs; ++k) HIP_TRY(hipGraphLaunch(g, c->stream));
-            i += 2 * imexlb_ctx::kLongPairs * longs;
-            c->have_f = c->have_pu = false;
-        }
-        const int64_t pairs = (nsteps - i) / 2;
-        if (pairs > 0) {
-            hipGraphExec_t g;
-            int rc = step_graph(c, 1, &g);
-            if (rc) return rc;
-            for (int64_t k = 0; k < pairs; ++k) HIP_TRY(hipGraphLaunch(g, c->stream));
-            i += 2 * pairs;
+            for (int64_t k = 0; k < reps; ++k) HIP_TRY(hipGraphLaunch(g, c->stream));
+            i += 2 * per * reps;
             c->have_f = c->have_pu = false;
         }
     }

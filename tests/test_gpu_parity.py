@@ -185,12 +185,12 @@ def test_ellipse_bitwise(aspect, angle, r_vec):
 
 
 def test_graph_replay_matches_oracle(r_vec):
-    """advance() replays pairs of fused steps as a HIP graph; after every
-    batch, with eager Monitor and download calls in between, the state equals
-    the oracle's bit for bit."""
+    """advance() replays pairs of fused steps as HIP graphs of 64, 8 and 1
+    pairs, largest first; after every call, with eager Monitor and download
+    calls in between, the state equals the oracle's bit for bit."""
     lat, o, d = _run_pair("cyl400x100", r_vec, 0)
     total = 0
-    for n in (1, 6, 7, 40, 3, 64):
+    for n in (1, 6, 7, 40, 3, 64, 301, 128):
         lat.advance(n)
         o.run(total, total + n - 1, 0)
         total += n

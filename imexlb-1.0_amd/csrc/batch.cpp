@@ -48,7 +48,11 @@ struct imexlb_batch {
 
     int block = 256, cpt = 1, persist = 0;
     unsigned persist_grid = 1024;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // captured pairs of steps, kGraphPairs[j] pairs per graph, by size and
+    // starting buffer; advance replays the larger first
+    static constexpr int kGraphSizes = 2;
+    static constexpr int kGraphPairs[kGraphSizes] = {16, 1};
+    hipGraphExec_t graph[kGraphSizes][2] = {};
     bool graphs = true;  // replay captured pairs of steps (IMEXLB_GRAPHS=0: launch every step)
 
     int cur = 1, lazy = -1;
@@ -56,11 +60,12 @@ struct imexlb_batch {
 
     void drop_graphs()
     {
-        for (auto &g : graph)
-            if (g) {
-                (void)hipGraphExecDestroy(g);
-                g = nullptr;
-            }
+        for (auto &set : graph)
+            for (auto &g : set)
+                if (g) {
+                    (void)hipGraphExecDestroy(g);
+                    g = nullptr;
+                }
     }
 
     void release()
@@ -137,15 +142,19 @@ struct imexlb_batch {
         return IMEXLB_OK;
     }
 
-    int pair_graph(hipGraphExec_t *out)
+    int pair_graph(int size, hipGraphExec_t *out)
     {
-        if (graph[cur]) {
-            *out = graph[cur];
+        hipGraphExec_t &slot = graph[size][cur];
+        if (slot) {
+            *out = slot;
             return IMEXLB_OK;
         }
         HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        hipError_t e = enqueue_step(cur);
-        if (e == hipSuccess) e = enqueue_step(1 - cur);
+        hipError_t e = hipSuccess;
+        for (int k = 0; k < kGraphPairs[size] && e == hipSuccess; ++k) {
+            e = enqueue_step(cur);
+            if (e == hipSuccess) e = enqueue_step(1 - cur);
+        }
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(stream, &g);
         if (e != hipSuccess || e2 != hipSuccess) {
@@ -156,7 +165,7 @@ struct imexlb_batch {
         e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) return fail(IMEXLB_E_HIP, std::string("batch graph: ") + hipGetErrorString(e));
-        graph[cur] = x;
+        slot = x;
         *out = x;
         return IMEXLB_OK;
     }
@@ -435,11 +444,14 @@ int imexlb_batch_advance(imexlb_batch *b, int64_t nsteps)
         return IMEXLB_OK;
     }
     int64_t pairs = nsteps / 2;
-    if (pairs > 0) {
+    for (int j = 0; j < imexlb_batch::kGraphSizes && pairs > 0; ++j) {
+        const int64_t reps = pairs / imexlb_batch::kGraphPairs[j];
+        if (reps <= 0) continue;
         hipGraphExec_t g;
-        int rc = b->pair_graph(&g);
+        int rc = b->pair_graph(j, &g);
         if (rc) return rc;
-        for (int64_t k = 0; k < pairs; ++k) HIP_TRY(hipGraphLaunch(g, b->stream));
+        for (int64_t k = 0; k < reps; ++k) HIP_TRY(hipGraphLaunch(g, b->stream));
+        pairs -= reps * imexlb_batch::kGraphPairs[j];
         // a pair returns to the same buffers; its second step streamed from 1-cur
         b->lazy = 1 - b->cur;
         b->have_pu = false;

@@ -184,6 +184,59 @@ def test_ellipse_bitwise(aspect, angle, r_vec):
     o.close()
 
 
+def test_phase_order_errors(r_vec):
+    """The phase state machine refuses calls out of the reference's order
+    (main.f90:44-61: SetGeometry, InitUP, InitPDF, then Collision,
+    BoundaryCondition, Propagation, PostProcessing) with IMEXLB_E_STATE, a
+    negative step count with IMEXLB_E_ARG, and a refused call changes
+    nothing: the run carried on after them equals the oracle bit for bit."""
+    deck = get_deck("tiny2d")
+    d = I.derive(deck)
+    print("[parity] phase-order errors on tiny2d", flush=True)
+    lat = I.Lattice(2, *deck.n[:2], device=0)
+    lat.set_params(deck.ome, deck.rho0, d.uu)
+
+    def refused(call, code=I.E_STATE):
+        with pytest.raises(I.ImexlbError) as e:
+            call()
+        assert e.value.code == code, (e.value.code, code)
+
+    refused(lambda: lat.init_up(r_vec))      # before SetGeometry
+    lat.set_geometry(d.pos, d.radius)
+    refused(lat.monitor)                     # no fields yet
+    refused(lat.init_pdf)                    # before InitUP
+    refused(lat.collision)
+    lat.init_up(r_vec)
+    refused(lat.collision)                   # before InitPDF
+    lat.init_pdf()
+    lat.set_fused(False)
+    refused(lat.boundary_condition)          # before Collision
+    refused(lat.propagation)
+    refused(lat.post_processing)
+    lat.collision()
+    refused(lat.collision)                   # twice
+    refused(lat.propagation)                 # before BoundaryCondition
+    lat.boundary_condition()
+    refused(lat.post_processing)             # before Propagation
+    lat.propagation()
+    refused(lambda: lat.set_fused(True))     # between Propagation and PostProcessing
+    refused(lambda: lat.advance(-1), I.E_ARG)
+    lat.post_processing()
+    lat.set_fused(True)
+    lat.advance(20)
+    lat.collision()
+    lat.boundary_condition()
+    lat.propagation()
+    refused(lambda: lat.set_fused(False))    # fused step pending
+    lat.post_processing()
+    o, _ = oracle_setup(deck, r_vec)
+    o.run(0, 21, 0)
+    assert lat.monitor() == o.monitor()
+    assert_state_equal(lat, o)
+    lat.close()
+    o.close()
+
+
 def test_graph_replay_matches_oracle(r_vec):
     """advance() replays pairs of fused steps as HIP graphs of 64, 8 and 1
     pairs, largest first; after every call, with eager Monitor and download

@@ -229,6 +229,15 @@ const char *imexlb_strerror(int code)
 
 const char *imexlb_last_error(void) { return last_error().c_str(); }
 
+// Launches the fused steps the phase calls deferred (defined with the graph
+// replay below); every entry point that launches other work or reads the
+// device state runs it first.
+static int flush_deferred(imexlb_ctx *c);
+#define FLUSH(c)                                            \
+    do {                                                    \
+        if (int rf_ = flush_deferred(c)) return rf_;        \
+    } while (0)
+
 int imexlb_device_count(int *n)
 {
     if (!n) return fail(IMEXLB_E_ARG, "null pointer");
@@ -351,6 +360,19 @@ int imexlb_set_params(imexlb_ctx *c, double ome, double rho0, double uu)
     CHECK_CTX(c);
     if (!std::isfinite(ome) || !std::isfinite(rho0) || !std::isfinite(uu) || rho0 == 0.0)
         return fail(IMEXLB_E_ARG, "non-finite parameter or rho0 == 0");
+    FLUSH(c);  // deferred steps keep the parameters they were asked with
+    if (!c->host_only && c->stage == S_FUSED && (ome != c->ome || rho0 != c->rho0 || uu != c->uu)) {
+        // The fused kernel of the last step has already relaxed the next one
+        // with the old parameters, where the reference's Collision reads
+        // them when it runs (lbm.f90:52-76).  Rebuild f, p, u of the current
+        // step (PostProcessing with the old parameters) and continue through
+        // the per-phase path: the next Collision relaxes them anew.
+        HIP_TRY(hipSetDevice(c->device));
+        int rc = c->materialize();
+        if (rc) return rc;
+        c->lazy_src = -1;
+        c->stage = S_PRE;
+    }
     c->ome = c->kp.ome = ome;
     c->rho0 = c->kp.rho0 = rho0;
     c->kp.inv_rho0 = 1.0 / rho0;
@@ -375,6 +397,7 @@ int imexlb_set_geometry(imexlb_ctx *c, double pos, double radius)
 int imexlb_set_geometry_shape(imexlb_ctx *c, double pos, double radius, double aspect, double angle_deg)
 {
     CHECK_CTX(c);
+    FLUSH(c);
     if (!std::isfinite(pos) || !std::isfinite(radius) || !std::isfinite(aspect) || !std::isfinite(angle_deg))
         return fail(IMEXLB_E_ARG, "non-finite geometry");
     if (aspect <= 0.0) return fail(IMEXLB_E_ARG, "ellipse axis ratio must be positive");
@@ -512,6 +535,7 @@ int imexlb_bc_table(imexlb_ctx *c, int64_t *slot, int64_t *src, int64_t cap, int
 int imexlb_init_up(imexlb_ctx *c, const double *r_vec)
 {
     CHECK_DEV(c);
+    FLUSH(c);
     if (!r_vec) return fail(IMEXLB_E_ARG, "null r_vec");
     if (!c->geom_set) return fail(IMEXLB_E_STATE, "SetGeometry before InitUP");
     HIP_TRY(hipMemcpyAsync(c->rv, r_vec, 100 * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -531,6 +555,7 @@ int imexlb_init_fields_f32(imexlb_ctx *c, const float *fields, int H, int W, dou
                            const double *r_vec)
 {
     CHECK_DEV(c);
+    FLUSH(c);
     if (c->dim != 2) return fail(IMEXLB_E_ARG, "FP32 field stacks are for 2D lattices");
     if (!fields || H <= 0 || W <= 0) return fail(IMEXLB_E_ARG, "fields");
     if (!c->geom_set) return fail(IMEXLB_E_STATE, "SetGeometry before setting fields");
@@ -551,6 +576,7 @@ int imexlb_init_fields_f32(imexlb_ctx *c, const float *fields, int H, int W, dou
 int imexlb_body_mask_f32(imexlb_ctx *c, float *out, int H, int W)
 {
     CHECK_DEV(c);
+    FLUSH(c);
     if (c->dim != 2) return fail(IMEXLB_E_ARG, "body masks are for 2D lattices");
     if (!out || H <= 0 || W <= 0 || H > c->kp.ny + 1 || W > c->kp.nx + 1) return fail(IMEXLB_E_ARG, "mask size");
     if (!c->geom_set) return fail(IMEXLB_E_STATE, "SetGeometry before the body mask");
@@ -561,6 +587,7 @@ int imexlb_body_mask_f32(imexlb_ctx *c, float *out, int H, int W)
 int imexlb_upload_fields(imexlb_ctx *c, const double *p, const double *u, int layout)
 {
     CHECK_DEV(c);
+    FLUSH(c);
     if (!p || !u) return fail(IMEXLB_E_ARG, "null field");
     if (layout != IMEXLB_LAYOUT_REF && layout != IMEXLB_LAYOUT_INTERIOR) return fail(IMEXLB_E_ARG, "layout");
     if (!c->geom_set) return fail(IMEXLB_E_STATE, "SetGeometry before upload");
@@ -592,6 +619,7 @@ int imexlb_upload_fields(imexlb_ctx *c, const double *p, const double *u, int la
 int imexlb_init_pdf(imexlb_ctx *c)
 {
     CHECK_DEV(c);
+    FLUSH(c);
     if (!c->have_pu || c->lazy_src >= 0) {
         int rc = c->materialize();
         if (rc) return rc;
@@ -617,6 +645,7 @@ int imexlb_collision(imexlb_ctx *c)
         return IMEXLB_OK;
     }
     if (c->stage != S_PRE) return fail(IMEXLB_E_STATE, "Collision needs InitPDF or a completed step");
+    FLUSH(c);
     // write the buffer the previous step did not use: its body cells are
     // the previous values BoundaryCondition may read
     StepArgs a = c->args();
@@ -636,6 +665,7 @@ int imexlb_boundary_condition(imexlb_ctx *c)
     CHECK_DEV(c);
     if (c->stage != S_COLLIDED) return fail(IMEXLB_E_STATE, "BoundaryCondition must follow Collision");
     if (!c->prefilled) {
+        FLUSH(c);
         int rc = c->run_fill(c->cur);
         if (rc) return rc;
     }
@@ -652,6 +682,7 @@ int imexlb_propagation(imexlb_ctx *c)
         c->stage = S_PENDING;
         return IMEXLB_OK;
     }
+    FLUSH(c);
     int rc = c->run_step(K_WRITE_F, c->fb[c->cur], nullptr, c->f);
     if (rc) return rc;
     c->have_f = true;
@@ -664,14 +695,22 @@ int imexlb_post_processing(imexlb_ctx *c)
 {
     CHECK_DEV(c);
     if (c->stage == S_PENDING) {
-        int rc = c->run_step(c->collide_mode(), c->fb[c->cur], c->fb[1 - c->cur], nullptr);
-        if (rc) return rc;
+        if (c->can_defer()) {  // launched later, as a graph replay with its neighbours
+            if (c->deferred == 0) c->defer_src = c->cur;
+            ++c->deferred;
+        } else {
+            FLUSH(c);
+            int rc = c->run_step(c->collide_mode(), c->fb[c->cur], c->fb[1 - c->cur], nullptr);
+            if (rc) return rc;
+        }
         c->lazy_src = c->cur;
         c->have_f = c->have_pu = false;
         c->stage = S_FUSED;
+        if (c->deferred >= imexlb_ctx::kDeferChunk) FLUSH(c);
         return IMEXLB_OK;
     }
     if (c->stage != S_STREAMED) return fail(IMEXLB_E_STATE, "PostProcessing must follow Propagation");
+    FLUSH(c);
     int rc = c->run_step(K_NOSHIFT | K_WRITE_PU, c->f, nullptr, nullptr);
     if (rc) return rc;
     c->have_pu = true;
@@ -682,6 +721,7 @@ int imexlb_post_processing(imexlb_ctx *c)
 int imexlb_set_fused(imexlb_ctx *c, int fused)
 {
     CHECK_CTX(c);
+    FLUSH(c);
     if (c->stage == S_PENDING || c->stage == S_STREAMED)
         return fail(IMEXLB_E_STATE, "cannot switch between Propagation and PostProcessing");
     c->fused = fused != 0;
@@ -703,9 +743,8 @@ static int one_step(imexlb_ctx *c)
 // the state is again S_FUSED with cur == X, so the graph replays.
 // Only kernels are recorded (no memset or copy nodes).  size: index into
 // kGraphPairs (the graph's number of pairs).
-static int step_graph(imexlb_ctx *c, int size, hipGraphExec_t *out)
+static int step_graph(imexlb_ctx *c, int size, int X, hipGraphExec_t *out)
 {
-    const int X = c->cur;
     const int pairs = imexlb_ctx::kGraphPairs[size];
     hipGraphExec_t &slot = c->graph[size][X];
     if (slot) {
@@ -742,13 +781,62 @@ static int step_graph(imexlb_ctx *c, int size, hipGraphExec_t *out)
     return IMEXLB_OK;
 }
 
+// n fused steps, the first reading fb[A]: one alone if n is odd, then pairs
+// as replays of the largest graphs that fit (the state they leave is the
+// one n launches one by one would)
+static int run_fused_steps(imexlb_ctx *c, int A, int64_t n)
+{
+    if (n & 1) {
+        int rc = c->run_step(c->collide_mode(), c->fb[A], c->fb[1 - A], nullptr);
+        if (rc) return rc;
+        A = 1 - A;
+        --n;
+    }
+    int64_t pairs = n / 2;
+    for (int j = 0; j < imexlb_ctx::kGraphSizes && pairs > 0; ++j) {
+        const int64_t reps = pairs / imexlb_ctx::kGraphPairs[j];
+        if (reps <= 0) continue;
+        hipGraphExec_t g;
+        int rc = step_graph(c, j, 1 - A, &g);  // from cur == X a pair first reads fb[1 - X]
+        if (rc) return rc;
+        for (int64_t k = 0; k < reps; ++k) HIP_TRY(hipGraphLaunch(g, c->stream));
+        pairs -= reps * imexlb_ctx::kGraphPairs[j];
+    }
+    return IMEXLB_OK;
+}
+
+static int flush_deferred(imexlb_ctx *c)
+{
+    if (c->deferred == 0) return IMEXLB_OK;
+    const int64_t n = c->deferred;
+    c->deferred = 0;
+    HIP_TRY(hipSetDevice(c->device));
+    return run_fused_steps(c, c->defer_src, n);
+}
+
 int imexlb_advance(imexlb_ctx *c, int64_t nsteps)
 {
     CHECK_DEV(c);
     if (nsteps < 0) return fail(IMEXLB_E_ARG, "negative step count");
     int64_t i = 0;
     // timed (profiled) runs launch kernel by kernel so each one is bracketed
-    if (c->fused && c->graphs && !c->prof) {
+    if (c->can_defer()) {
+        for (; i < nsteps && c->stage != S_FUSED; ++i) {
+            int rc = one_step(c);
+            if (rc) return rc;
+        }
+        if (i < nsteps) {  // the rest joins the deferred steps: from cur == X the next step reads fb[1 - X]
+            if (c->deferred == 0) c->defer_src = 1 - c->cur;
+            c->deferred += nsteps - i;
+            c->cur ^= int((nsteps - i) & 1);  // each step's Collision flips it
+            c->lazy_src = c->cur;
+            c->have_f = c->have_pu = false;
+            i = nsteps;
+        }
+        return flush_deferred(c);  // advance launches everything it was asked for
+    }
+    FLUSH(c);
+    if (c->fused && c->graphs && !c->prof) {  // a boundary fill before every pull: graphs of whole pairs
         for (; i < nsteps && c->stage != S_FUSED; ++i) {
             int rc = one_step(c);
             if (rc) return rc;
@@ -758,7 +846,7 @@ int imexlb_advance(imexlb_ctx *c, int64_t nsteps)
             const int64_t reps = (nsteps - i) / (2 * per);
             if (reps <= 0) continue;
             hipGraphExec_t g;
-            int rc = step_graph(c, j, &g);
+            int rc = step_graph(c, j, c->cur, &g);
             if (rc) return rc;
             for (int64_t k = 0; k < reps; ++k) HIP_TRY(hipGraphLaunch(g, c->stream));
             i += 2 * per * reps;
@@ -775,6 +863,7 @@ int imexlb_advance(imexlb_ctx *c, int64_t nsteps)
 int imexlb_monitor(imexlb_ctx *c, double *um)
 {
     CHECK_DEV(c);
+    FLUSH(c);
     if (!um) return fail(IMEXLB_E_ARG, "null pointer");
     int rc = c->materialize();
     if (rc) return rc;
@@ -797,6 +886,7 @@ int imexlb_monitor(imexlb_ctx *c, double *um)
 int imexlb_forces(imexlb_ctx *c, double F[3])
 {
     CHECK_DEV(c);
+    FLUSH(c);
     if (!F) return fail(IMEXLB_E_ARG, "null pointer");
     const void *buf;
     if (c->stage == S_FUSED)
@@ -823,6 +913,7 @@ int imexlb_forces(imexlb_ctx *c, double F[3])
 int imexlb_mass(imexlb_ctx *c, double *m)
 {
     CHECK_DEV(c);
+    FLUSH(c);
     if (!m) return fail(IMEXLB_E_ARG, "null pointer");
     if (c->stage == S_NONE) return fail(IMEXLB_E_STATE, "no populations yet");
     int rc = c->materialize();
@@ -840,6 +931,7 @@ int imexlb_mass(imexlb_ctx *c, double *m)
 int imexlb_download_fields(imexlb_ctx *c, double *p, double *u, int layout)
 {
     CHECK_DEV(c);
+    FLUSH(c);
     if (layout != IMEXLB_LAYOUT_REF && layout != IMEXLB_LAYOUT_INTERIOR) return fail(IMEXLB_E_ARG, "layout");
     int rc = c->materialize();
     if (rc) return rc;
@@ -868,6 +960,7 @@ int imexlb_download_fields(imexlb_ctx *c, double *p, double *u, int layout)
 int imexlb_download_pdf(imexlb_ctx *c, int which, double *out, int layout)
 {
     CHECK_DEV(c);
+    FLUSH(c);
     if (!out) return fail(IMEXLB_E_ARG, "null output");
     if (layout != IMEXLB_LAYOUT_REF && layout != IMEXLB_LAYOUT_INTERIOR) return fail(IMEXLB_E_ARG, "layout");
     const void *src;
@@ -911,6 +1004,7 @@ int imexlb_download_pdf(imexlb_ctx *c, int which, double *out, int layout)
 int imexlb_synchronize(imexlb_ctx *c)
 {
     CHECK_DEV(c);
+    FLUSH(c);
     HIP_TRY(hipStreamSynchronize(c->stream));
     return IMEXLB_OK;
 }
@@ -918,6 +1012,7 @@ int imexlb_synchronize(imexlb_ctx *c)
 int imexlb_profile(imexlb_ctx *c, int enable)
 {
     CHECK_DEV(c);
+    FLUSH(c);
     HIP_TRY(hipStreamSynchronize(c->stream));
     for (auto &t : c->timed) {
         (void)hipEventDestroy(t.a);

@@ -127,6 +127,17 @@ struct imexlb_ctx {
                 }
     }
 
+    // Fused steps asked for through the phase calls (PostProcessing of a
+    // lattice whose step writes its own boundary copies) and not launched
+    // yet: they run as graph replays once kDeferChunk are pending (one graph
+    // of kDeferChunk / 2 pairs, so the GPU works while the caller goes on)
+    // and when the state is next needed (any other call that launches work
+    // or reads device memory).  Step k of them reads fb[defer_src ^ (k & 1)].
+    int64_t deferred = 0;
+    int defer_src = 0;
+    static constexpr int64_t kDeferChunk = 16;
+    bool can_defer() const { return fused && graphs && !prof && push_on; }
+
     // profiling
     bool prof = false;
     std::vector<TimedLaunch> timed;

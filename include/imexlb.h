@@ -144,7 +144,13 @@ int imexlb_body_mask_f32(imexlb_ctx *ctx, float *out, int H, int W);
 /* InitPDF (lbm.f90:24-48): f = feq(p, u) on fluid nodes. */
 int imexlb_init_pdf(imexlb_ctx *ctx);
 
-/* The four phases of one time step, main.f90:57-61. */
+/* The four phases of one time step, main.f90:57-61.  In fused mode, on a
+ * lattice whose step writes its own boundary copies, the step's kernel is
+ * queued rather than launched by PostProcessing: queued steps run as graph
+ * replays once 16 are pending and before any other call that launches work
+ * or reads device state (so results are the same; an error of such a launch
+ * is returned by the call that runs it).  set_params after a fused step
+ * relaxes the next Collision with the new parameters, as the reference. */
 int imexlb_collision(imexlb_ctx *ctx);           /* Collision          lbm.f90:52-76   */
 int imexlb_boundary_condition(imexlb_ctx *ctx);  /* BoundaryCondition  lbm.f90:96-155,
                                                     incl. PassF        cart_mpi.f90:240 */

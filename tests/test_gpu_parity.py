@@ -237,6 +237,57 @@ def test_phase_order_errors(r_vec):
     o.close()
 
 
+def _phase_steps(lat, n):
+    """n steps through the reference's four phase calls (main.f90:57-61)."""
+    for _ in range(n):
+        lat.collision()
+        lat.boundary_condition()
+        lat.propagation()
+        lat.post_processing()
+
+
+def test_deferred_phase_steps_match_oracle(r_vec):
+    """Through the four phase calls a fused lattice defers its steps and runs
+    them as graph replays when the state is next needed.  Interleaved with
+    Monitor, downloads, a forces call, advance() and a parameter change in
+    the middle of the run (the pending steps keep the old parameters; the
+    next Collision uses the new ones, as the reference's does), and over
+    many 16-step chunks through phase calls alone: equal to the oracle."""
+    deck = get_deck("tiny2d")
+    d = I.derive(deck)
+    print("[parity] deferred phase steps on tiny2d", flush=True)
+    lat, _ = I.setup(deck, r_vec, device=0)
+    o, _ = oracle_setup(deck, r_vec)
+
+    def check():
+        assert lat.monitor() == o.monitor()
+        assert_state_equal(lat, o)
+
+    _phase_steps(lat, 37)
+    o.run(0, 36, 0)
+    check()
+    _phase_steps(lat, 5)
+    o.run(37, 41, 0)
+    F, Fo = lat.forces(), o.forces()
+    assert np.all(np.abs(F - Fo) <= 1e-12 * np.abs(Fo).max()), (F, Fo)
+    _phase_steps(lat, 3)
+    lat.advance(12)  # joins the 3 pending steps
+    o.run(42, 56, 0)
+    check()
+    _phase_steps(lat, 9)
+    lat.set_params(1.7, deck.rho0, d.uu)  # the 9 pending steps run first, with the old ome
+    o.run(57, 65, 0)
+    o.set_params(1.7, deck.rho0, d.uu)
+    _phase_steps(lat, 11)
+    o.run(66, 76, 0)
+    check()
+    _phase_steps(lat, 1100)  # 68 chunks of 16 run on their own, the last 12 on Monitor
+    o.run(77, 1176, 0)
+    check()
+    lat.close()
+    o.close()
+
+
 def test_graph_replay_matches_oracle(r_vec):
     """advance() replays pairs of fused steps as HIP graphs of 64, 8 and 1
     pairs, largest first; after every call, with eager Monitor and download

@@ -309,10 +309,15 @@ int imexlb_create(imexlb_ctx **out, int dim, int nx, int ny, int nz, int device,
     k.back_tiles = uint32_t((1 + k.Px + k.PxPy + 63) / 64);
     k.rdy = c->g.dy;
     k.rdz = c->g.dz;
-    // XCD-contiguous block order pays for D3Q27 FP64 only (measured on
-    // MI355X); IMEXLB_XCD_REMAP=0/1 overrides
+    // XCD-contiguous block order (each XCD steps the same eighth of the
+    // lattice every step, so a step reads the previous one's output from its
+    // own L2) pays for D3Q27 FP64 and for the small 2D lattices that stay in
+    // the caches between steps (measured on MI355X); IMEXLB_XCD_REMAP=0/1/R
+    // overrides
+    if (const char *sv = std::getenv("IMEXLB_SMALL_CELLS")) c->small_cells = std::atoll(sv);
     const char *xv = std::getenv("IMEXLB_XCD_REMAP");
-    k.xcd_remap = xv ? uint32_t(std::max(0, std::atoi(xv))) : uint32_t(dim == 3 && precision == IMEXLB_F64);
+    k.xcd_remap = xv ? uint32_t(std::max(0, std::atoi(xv)))
+                     : uint32_t(dim == 3 ? precision == IMEXLB_F64 : int64_t(k.ncell) <= c->small_cells);
 
     if (const char *fv = std::getenv("IMEXLB_FAIL_ALLOC")) c->fail_alloc_at = std::atoll(fv);
     if (device >= 0) {
@@ -325,7 +330,6 @@ int imexlb_create(imexlb_ctx **out, int dim, int nx, int ny, int nz, int device,
         c->graphs = !(gv && gv[0] == '0');
         const char *pv = std::getenv("IMEXLB_PUSH");
         c->push_enabled = !(pv && pv[0] == '0');
-        if (const char *sv = std::getenv("IMEXLB_SMALL_CELLS")) c->small_cells = std::atoll(sv);
         HIP_TRY(hipSetDevice(device));
         HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         const size_t pop = size_t(c->Q) * size_t(c->plane) * c->ssz;

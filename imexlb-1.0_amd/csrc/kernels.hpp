@@ -94,9 +94,13 @@ inline int64_t pop_slot(int64_t q, int64_t c, int64_t plane, int Q)
 // latency-bound lattice -- every cell loads its push-table range together
 // with its populations instead of after its node type (4 more bytes per
 // cell), the output is written with ordinary (cached) stores, and blocks
-// have kSmallBlock threads (more of the CUs take part).
+// are one wave (more of the CUs take part; with the XCD-contiguous block
+// order each XCD keeps stepping the same eighth of the lattice).
 enum : int { K_COLLIDE = 1, K_WRITE_F = 2, K_WRITE_PU = 4, K_NOSHIFT = 8, K_PUSH = 16, K_IDX64 = 32, K_SMALL = 64 };
-constexpr unsigned kSmallBlock = 128;
+#ifndef IMEXLB_SMALL_BLOCK
+#define IMEXLB_SMALL_BLOCK 64
+#endif
+constexpr unsigned kSmallBlock = IMEXLB_SMALL_BLOCK;
 
 // node-type byte on the device: fluid node; node with boundary copies to push
 enum : uint8_t { G_FLUID = 1, G_PUSH = 2 };

@@ -170,8 +170,11 @@ int build_device_tables(const imexlb_ctx *c, DeviceTables &t)
             int x, y, z;
             g.coords(rs / c->Q, x, y, z);
             const uint32_t cell = uint32_t(c->pidx(x, y, z));
-            // this step's value comes from a fluid node, a previous one from a body cell
+            // this step's value comes from a fluid node, a previous one from a
+            // body cell, and either is an interior node (the step has no
+            // thread on ghost cells)
             if (bool(t.geo[cell] & G_FLUID) == prev) t.push_ok = false;
+            if (!g.interior(x, y, z)) t.push_ok = false;
             v.push_back({cell, uint8_t(rs % c->Q), dev_slot(e.ref_slot)});
         }
         std::sort(v.begin(), v.end(), [](const Copy &a, const Copy &b) {
@@ -180,10 +183,12 @@ int build_device_tables(const imexlb_ctx *c, DeviceTables &t)
         t.push_beg.assign(size_t(c->kp.ncell) + 1, 0);
         t.push_q.resize(v.size());
         t.push_dst.resize(v.size());
+        t.push_lds.resize(c->dim == 2 ? v.size() : 0);
         for (size_t k = 0; k < v.size(); ++k) {
             ++t.push_beg[size_t(v[k].cell) + 1];
             t.push_q[k] = v[k].q;
             t.push_dst[k] = v[k].dst;
+            if (c->dim == 2) t.push_lds[k] = uint16_t(v[k].q * 64u + (v[k].cell & 63u));
             t.geo[v[k].cell] |= G_PUSH;
         }
         for (size_t k = 1; k < t.push_beg.size(); ++k) t.push_beg[k] += t.push_beg[k - 1];
@@ -437,7 +442,7 @@ int imexlb_set_geometry_shape(imexlb_ctx *c, double pos, double radius, double a
     HIP_TRY(hipMemcpy(c->geo, t.geo.data(), t.geo.size(), hipMemcpyHostToDevice));
     for (void *q : {(void *)c->fill_dst, (void *)c->fill_src, (void *)c->link_out, (void *)c->link_back,
                     (void *)c->link_q, (void *)c->force_partial, (void *)c->force_out, (void *)c->push_beg,
-                    (void *)c->push_q, c->push_dst})
+                    (void *)c->push_q, c->push_dst, (void *)c->push_lds})
         if (q) (void)hipFree(q);
     c->fill_dst = c->fill_src = c->link_out = c->link_back = nullptr;
     c->link_q = nullptr;
@@ -445,6 +450,7 @@ int imexlb_set_geometry_shape(imexlb_ctx *c, double pos, double radius, double a
     c->push_beg = nullptr;
     c->push_q = nullptr;
     c->push_dst = nullptr;
+    c->push_lds = nullptr;
     // boundary slots whose value is 0 are never written by the pushed copies:
     // start both buffers from 0 for this geometry
     const size_t pop = size_t(c->Q) * size_t(c->plane) * c->ssz;
@@ -467,11 +473,14 @@ int imexlb_set_geometry_shape(imexlb_ctx *c, double pos, double radius, double a
     c->push_on = c->push_enabled && t.push_ok;
     const size_t np = std::max<size_t>(t.push_q.size(), 1);
     if ((rc = dev_alloc(c, (void **)&c->push_beg, t.push_beg.size() * sizeof(int32_t))) ||
-        (rc = dev_alloc(c, (void **)&c->push_q, np)) || (rc = dev_alloc(c, &c->push_dst, np * isz)))
+        (rc = dev_alloc(c, (void **)&c->push_q, np)) || (rc = dev_alloc(c, &c->push_dst, np * isz)) ||
+        (c->dim == 2 && (rc = dev_alloc(c, (void **)&c->push_lds, np * sizeof(uint16_t)))))
         return rc;
     HIP_TRY(hipMemcpy(c->push_beg, t.push_beg.data(), t.push_beg.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     if (!t.push_q.empty()) HIP_TRY(hipMemcpy(c->push_q, t.push_q.data(), t.push_q.size(), hipMemcpyHostToDevice));
     HIP_TRY(upload_slots(c->push_dst, t.push_dst, c->fill32));
+    if (!t.push_lds.empty())
+        HIP_TRY(hipMemcpy(c->push_lds, t.push_lds.data(), t.push_lds.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
     if (c->nlink) {
         HIP_TRY(hipMemcpy(c->link_out, t.link_out.data(), t.link_out.size() * 8, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(c->link_back, t.link_back.data(), t.link_back.size() * 8, hipMemcpyHostToDevice));

@@ -72,6 +72,7 @@ struct imexlb_ctx {
     int32_t *push_beg = nullptr;
     uint8_t *push_q = nullptr;
     void *push_dst = nullptr;   // int32 (fill32) or int64 slots
+    uint16_t *push_lds = nullptr;  // 2D: LDS position of each entry's value
     bool push_enabled = true;   // IMEXLB_PUSH=0 keeps the separate fill
     bool push_on = false;       // enabled and the table qualifies
     bool prefilled = false;     // buffer cur already holds its boundary copies
@@ -159,6 +160,7 @@ struct imexlb_ctx {
         a.push.beg = push_beg;
         a.push.q = push_q;
         a.push.dst = push_dst;
+        a.push.lds = push_lds;
         return a;
     }
 
@@ -270,13 +272,14 @@ struct imexlb_ctx {
         timed.clear();
         void *ptrs[] = {fb[0],     fb[1],  f,         p,        ux,       uy,        uz,     rv,
                         geo,       fill_dst, fill_src, red,     link_out, link_back, link_q, force_partial,
-                        force_out, push_beg, push_q,   push_dst, mass_partial};
+                        force_out, push_beg, push_q,   push_dst, push_lds, mass_partial};
         for (void *q : ptrs)
             if (q) (void)hipFree(q);
         fb[0] = fb[1] = f = nullptr;
         push_beg = nullptr;
         push_q = nullptr;
         push_dst = nullptr;
+        push_lds = nullptr;
         p = ux = uy = uz = rv = nullptr;
         geo = nullptr;
         fill_dst = fill_src = nullptr;
@@ -320,6 +323,7 @@ struct DeviceTables {
     std::vector<int32_t> push_beg;  // ncell + 1 offsets
     std::vector<uint8_t> push_q;
     std::vector<int64_t> push_dst;
+    std::vector<uint16_t> push_lds;  // 2D lattices only
     bool push_ok = true;  // every copy's source is a fluid node or a body cell
     std::vector<int64_t> link_out, link_back;
     std::vector<int8_t> link_q;

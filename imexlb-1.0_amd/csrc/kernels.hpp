@@ -91,16 +91,12 @@ inline int64_t pop_slot(int64_t q, int64_t c, int64_t plane, int Q)
 // K_PUSH: the collide step also writes the boundary copies of its output
 // (PushTable), so the buffer needs no boundary fill; K_IDX64: the push
 // table's destination slots are int64 (else int32); K_SMALL: a small,
-// latency-bound lattice -- every cell loads its push-table range together
-// with its populations instead of after its node type (4 more bytes per
-// cell), the output is written with ordinary (cached) stores, and blocks
-// are one wave (more of the CUs take part; with the XCD-contiguous block
-// order each XCD keeps stepping the same eighth of the lattice).
+// latency-bound 2D lattice -- blocks of one wave, each a 64-cell tile whose
+// boundary copies are spread over its lanes through LDS, the output written
+// with ordinary (cached) stores (with the XCD-contiguous block order each
+// XCD keeps stepping the same eighth of the lattice, in its own L2).
 enum : int { K_COLLIDE = 1, K_WRITE_F = 2, K_WRITE_PU = 4, K_NOSHIFT = 8, K_PUSH = 16, K_IDX64 = 32, K_SMALL = 64 };
-#ifndef IMEXLB_SMALL_BLOCK
-#define IMEXLB_SMALL_BLOCK 64
-#endif
-constexpr unsigned kSmallBlock = IMEXLB_SMALL_BLOCK;
+constexpr unsigned kSmallBlock = 64;
 
 // node-type byte on the device: fluid node; node with boundary copies to push
 enum : uint8_t { G_FLUID = 1, G_PUSH = 2 };
@@ -113,6 +109,9 @@ struct PushTable {
     const int32_t *beg = nullptr;
     const uint8_t *q = nullptr;
     const void *dst = nullptr;  // int32 or int64 slots (K_IDX64)
+    // 2D lattices: q * 64 + (source cell % 64) of each entry, where the
+    // K_SMALL step finds the copied value in its wave's LDS
+    const uint16_t *lds = nullptr;
 };
 
 // One case of a batched ensemble: its parameters and where its populations

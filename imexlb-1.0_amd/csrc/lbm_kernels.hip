@@ -486,8 +486,7 @@ __device__ __forceinline__ void gather_wave(const S *__restrict__ src, const KPa
 }
 
 // Entries [b, e) of the push table that cell c copies out: empty unless
-// K_PUSH and the cell has copies.  K_SMALL reads the range of every
-// cell, so the load does not wait for the node type.
+// K_PUSH and the cell has copies.
 struct PushRange {
     int32_t b, e;
 };
@@ -495,7 +494,7 @@ template <int MODE>
 __device__ __forceinline__ PushRange push_range(const PushTable &pt, uint8_t g, uint32_t c)
 {
     PushRange r{0, 0};
-    if ((MODE & K_PUSH) && ((MODE & K_SMALL) || (g & G_PUSH))) {
+    if ((MODE & K_PUSH) && (g & G_PUSH)) {
         r.b = pt.beg[c];
         r.e = pt.beg[c + 1];
     }
@@ -617,7 +616,7 @@ __device__ __forceinline__ void finish_node(const C (&f)[Lat<D>::Q], uint8_t g, 
             S *o = (MODE & K_WAVE) ? wdst + (uint32_t(store_q(q, L::Q)) * 64u + (c & 63u)) : dst + cslot<L::Q>(kp, q, c);
             const S sv = Store<S>::template st<C>(v);
             if constexpr (kRegPush) out[q] = sv;
-            if (IMEXLB_STEP_NT && !(MODE & K_SMALL))
+            if (IMEXLB_STEP_NT)
                 nt_store(o, sv);
             else
                 *o = sv;
@@ -633,6 +632,73 @@ __device__ __forceinline__ void finish_node(const C (&f)[Lat<D>::Q], uint8_t g, 
             const int32_t e1 = pt.beg[c + 1];
             for (int32_t e = pt.beg[c]; e < e1; ++e) dst[push_slot<MODE>(pt, e)] = dst[cslot<L::Q>(kp, pt.q[e], c)];
         }
+    }
+}
+
+// K_SMALL step (D2Q9 lattices small enough to stay in the caches between
+// steps, where the time of a step is the time of its slowest wave): a block
+// is one wave and one 64-cell tile.  The push table is grouped by cell, so
+// the tile's boundary copies are one run [eb, ee) of it, found with two
+// loads at wave-uniform addresses that need nothing from the cells.  Every
+// lane of a tile with copies leaves its values in LDS (a fluid node its
+// output, a body cell its previous-step values), and the run is spread over
+// the wave's lanes: one LDS read and one store per copy, the first
+// kSmallRounds rounds of table entries loaded before the arithmetic.
+constexpr int kSmallRounds = 4;
+template <class S>
+using BitsOf = std::conditional_t<sizeof(S) == 8, uint64_t, std::conditional_t<sizeof(S) == 4, uint32_t, uint16_t>>;
+
+template <int D, class S, class C, int MODE>
+__device__ __forceinline__ void step_small(const S *__restrict__ src, S *__restrict__ dst,
+                                           const uint8_t *__restrict__ geo, const KParams &kp, const PushTable &pt,
+                                           uint32_t tile)
+{
+    static_assert(D == 2 && (MODE & K_PUSH) && !(MODE & K_IDX64), "K_SMALL: D2Q9 with int32 push slots");
+    static_assert(IMEXLB_CELLMAP && kSmallBlock == 64, "K_SMALL: one thread per padded cell, one-wave blocks");
+    using L = Lat<D>;
+    using U = BitsOf<S>;
+    __shared__ U sh[L::Q * 64];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t c0 = tile * 64u;
+    if (c0 >= kp.ncell) return;  // the whole block
+    const int32_t eb = __builtin_amdgcn_readfirstlane(pt.beg[c0]);
+    const int32_t ee = __builtin_amdgcn_readfirstlane(pt.beg[min(c0 + 64u, kp.ncell)]);
+    const int32_t *const pdst = static_cast<const int32_t *>(pt.dst);
+    uint32_t li[kSmallRounds] = {}, ds[kSmallRounds] = {};
+#pragma unroll
+    for (int r = 0; r < kSmallRounds; ++r) {
+        const int32_t e = eb + int32_t(lane) + 64 * r;
+        if (e < ee) {
+            li[r] = pt.lds[e];
+            ds[r] = uint32_t(pdst[e]);
+        }
+    }
+    int x, y, z;
+    uint32_t c;
+    if (decode_cell<D>(kp, c0 + lane, x, y, z, c)) {
+        const uint8_t g = geo[c];
+        C f[L::Q];
+        gather<D, S, C, MODE>(src, kp, c, f);
+        if (g & G_FLUID) {
+            C p, ux, uy, uz;
+            moments<D, C>(f, kp, x, y, z, p, ux, uy, uz);
+            collide<D, C>(f, p, ux, uy, uz, kp, [&](int q, C v) {
+                const S sv = Store<S>::template st<C>(v);
+                dst[cslot<L::Q>(kp, q, c)] = sv;
+                if (eb < ee) sh[q * 64 + int(lane)] = __builtin_bit_cast(U, sv);
+            });
+        } else if (g & G_PUSH) {
+#pragma unroll
+            for (int q = 0; q < L::Q; ++q) sh[q * 64 + int(lane)] = __builtin_bit_cast(U, src[cslot<L::Q>(kp, q, c)]);
+        }
+    }
+    if (eb < ee) {  // the same for every lane of the block
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kSmallRounds; ++r)
+            if (eb + int32_t(lane) + 64 * r < ee) dst[ds[r]] = __builtin_bit_cast(S, sh[li[r]]);
+        for (int32_t e = eb + int32_t(lane) + 64 * kSmallRounds; e < ee; e += 64)
+            dst[uint32_t(pdst[e])] = __builtin_bit_cast(S, sh[pt.lds[e]]);
     }
 }
 
@@ -660,7 +726,9 @@ __global__ IMEXLB_STEP_BOUNDS void k_step(const S *__restrict__ src, S *__restri
 {
     constexpr int CPT = step_cpt<MODE>();
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, kp.xcd_remap);
-    if constexpr (CPT == 1) {
+    if constexpr ((MODE & K_SMALL) != 0) {
+        step_small<D, S, C, MODE>(src, dst, geo, kp, pt, blk);
+    } else if constexpr (CPT == 1) {
         int x, y, z;
         uint32_t c;
         const uint32_t i = blk * blockDim.x + threadIdx.x;
@@ -1076,7 +1144,9 @@ static hipError_t step_d(int prec, int mode, const StepArgs &a, hipStream_t s)
     case K_COLLIDE: return step_dm<D, K_COLLIDE>(prec, a, s);
     case K_COLLIDE | K_PUSH: return step_dm<D, K_COLLIDE | K_PUSH>(prec, a, s);
     case K_COLLIDE | K_PUSH | K_IDX64: return step_dm<D, K_COLLIDE | K_PUSH | K_IDX64>(prec, a, s);
-    case K_COLLIDE | K_PUSH | K_SMALL: return step_dm<D, K_COLLIDE | K_PUSH | K_SMALL>(prec, a, s);
+    case K_COLLIDE | K_PUSH | K_SMALL:  // the small-lattice form is 2D only
+        if constexpr (D == 2) return step_dm<D, K_COLLIDE | K_PUSH | K_SMALL>(prec, a, s);
+        return hipErrorInvalidValue;
     case K_WRITE_F: return step_dm<D, K_WRITE_F>(prec, a, s);
     case K_WRITE_F | K_WRITE_PU: return step_dm<D, K_WRITE_F | K_WRITE_PU>(prec, a, s);
     case K_NOSHIFT | K_WRITE_PU: return step_dm<D, K_NOSHIFT | K_WRITE_PU>(prec, a, s);

@@ -643,7 +643,10 @@ __device__ __forceinline__ void finish_node(const C (&f)[Lat<D>::Q], uint8_t g, 
 // lane of a tile with copies leaves its values in LDS (a fluid node its
 // output, a body cell its previous-step values), and the run is spread over
 // the wave's lanes: one LDS read and one store per copy, the first
-// kSmallRounds rounds of table entries loaded before the arithmetic.
+// kSmallRounds rounds of table entries (and in tiles with copies every
+// cell's own values) loaded before the arithmetic.  Taking the entries into
+// registers before the node's output stores, or storing the outputs after
+// the arithmetic, measured slower.
 constexpr int kSmallRounds = 4;
 template <class S>
 using BitsOf = std::conditional_t<sizeof(S) == 8, uint64_t, std::conditional_t<sizeof(S) == 4, uint32_t, uint16_t>>;
@@ -679,6 +682,11 @@ __device__ __forceinline__ void step_small(const S *__restrict__ src, S *__restr
         const uint8_t g = geo[c];
         C f[L::Q];
         gather<D, S, C, MODE>(src, kp, c, f);
+        U own[L::Q];
+        if (eb < ee) {  // what a body cell copies, loaded with the populations
+#pragma unroll
+            for (int q = 0; q < L::Q; ++q) own[q] = __builtin_bit_cast(U, src[cslot<L::Q>(kp, q, c)]);
+        }
         if (g & G_FLUID) {
             C p, ux, uy, uz;
             moments<D, C>(f, kp, x, y, z, p, ux, uy, uz);
@@ -687,9 +695,9 @@ __device__ __forceinline__ void step_small(const S *__restrict__ src, S *__restr
                 dst[cslot<L::Q>(kp, q, c)] = sv;
                 if (eb < ee) sh[q * 64 + int(lane)] = __builtin_bit_cast(U, sv);
             });
-        } else if (g & G_PUSH) {
+        } else if (g & G_PUSH) {  // (a cell with copies: eb < ee)
 #pragma unroll
-            for (int q = 0; q < L::Q; ++q) sh[q * 64 + int(lane)] = __builtin_bit_cast(U, src[cslot<L::Q>(kp, q, c)]);
+            for (int q = 0; q < L::Q; ++q) sh[q * 64 + int(lane)] = own[q];
         }
     }
     if (eb < ee) {  // the same for every lane of the block

@@ -23,6 +23,10 @@ TINY = {
     # odd sizes, body off the half-integer centre line: 38x24 and 24x18x14
     "ragged2d": (2, "37 23\n0.2 0.3 20\n1.7 1 1\n0.02 0.001\n"),
     "ragged3d": (3, "23 17 13\n0.3 0.35 20\n1.55 1 1\n0.02 0.001\n"),
+    # three rows of 62 nodes: row y = 1 is the source of both walls'
+    # bounce-back copies, so one 64-cell tile of the device layout holds 379
+    # boundary copies (more than the small-lattice step loads up front)
+    "channel3": (2, "61 2\n0.02 0.5 2\n1.6 1 1\n0.02 0.001\n"),
 }
 
 

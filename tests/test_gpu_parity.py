@@ -41,7 +41,7 @@ def _advance_with_monitor(lat, o, d, steps):
 
 @pytest.mark.parametrize("name,steps", [("tiny2d", 70), ("tiny3d", 44), ("inletbody2d", 145),
                                         ("wallsphere3d", 52), ("ysphere3d", 40), ("outletsphere3d", 40),
-                                        ("ragged2d", 90), ("ragged3d", 40)])
+                                        ("ragged2d", 90), ("ragged3d", 40), ("channel3", 60)])
 @pytest.mark.parametrize("fused", [True, False])
 def test_full_run_bitwise_small(name, steps, fused, r_vec):
     lat, o, d = _run_pair(name, r_vec, steps, fused)
@@ -109,12 +109,14 @@ def test_block_order_does_not_change_results(remap, r_vec, monkeypatch):
 @pytest.mark.parametrize("small_cells", ["0", str(1 << 30)])
 def test_small_lattice_mode_does_not_change_results(small_cells, r_vec, monkeypatch):
     """2D lattices up to IMEXLB_SMALL_CELLS padded cells step with the
-    small-lattice kernel form (every cell's copy range loaded with its
-    populations, cached stores, 128-thread blocks); forced off and on, the
-    decks with walls, a body cutting the inlet and the 400x100 cylinder
-    equal the oracle."""
+    small-lattice kernel form (one-wave blocks, each tile's boundary copies
+    spread over its lanes through LDS, cached stores); forced off and on,
+    the decks with walls, a body cutting the inlet, a 3-row channel whose
+    tile holds more copies than the kernel loads up front, and the 400x100
+    cylinder equal the oracle."""
     monkeypatch.setenv("IMEXLB_SMALL_CELLS", small_cells)
-    for name, steps in (("tiny2d", 61), ("inletbody2d", 70), ("ragged2d", 45), ("cyl400x100", 200)):
+    for name, steps in (("tiny2d", 61), ("inletbody2d", 70), ("ragged2d", 45), ("channel3", 50),
+                        ("cyl400x100", 200)):
         lat, o, d = _run_pair(name, r_vec, steps)
         lat.advance(steps)
         o.run(0, steps - 1, 0)

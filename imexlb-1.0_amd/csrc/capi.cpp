@@ -314,12 +314,17 @@ int imexlb_create(imexlb_ctx **out, int dim, int nx, int ny, int nz, int device,
     k.back_tiles = uint32_t((1 + k.Px + k.PxPy + 63) / 64);
     k.rdy = c->g.dy;
     k.rdz = c->g.dz;
+    // The small-lattice step form pays while the two population buffers stay
+    // in the 256 MB Infinity Cache from step to step: up to 240 MiB of them
+    // and 5 Mi cells (measured on MI355X from 400x100 to 8192x2048 in every
+    // storage form); IMEXLB_SMALL_CELLS overrides the cell limit.
+    c->small_cells = std::min<int64_t>(int64_t(5) << 20, (int64_t(240) << 20) / (2 * 9 * int64_t(c->ssz)));
+    if (const char *sv = std::getenv("IMEXLB_SMALL_CELLS")) c->small_cells = std::atoll(sv);
     // XCD-contiguous block order (each XCD steps the same eighth of the
     // lattice every step, so a step reads the previous one's output from its
     // own L2) pays for D3Q27 FP64 and for the small 2D lattices that stay in
     // the caches between steps (measured on MI355X); IMEXLB_XCD_REMAP=0/1/R
     // overrides
-    if (const char *sv = std::getenv("IMEXLB_SMALL_CELLS")) c->small_cells = std::atoll(sv);
     const char *xv = std::getenv("IMEXLB_XCD_REMAP");
     k.xcd_remap = xv ? uint32_t(std::max(0, std::atoi(xv)))
                      : uint32_t(dim == 3 ? precision == IMEXLB_F64 : int64_t(k.ncell) <= c->small_cells);

@@ -233,7 +233,7 @@ struct imexlb_ctx {
         if (!fill32) return K_COLLIDE | K_PUSH | K_IDX64;
         return K_COLLIDE | K_PUSH | (dim == 2 && int64_t(kp.ncell) <= small_cells ? K_SMALL : 0);
     }
-    int64_t small_cells = int64_t(1) << 18;  // IMEXLB_SMALL_CELLS
+    int64_t small_cells = 0;  // set at creation (IMEXLB_SMALL_CELLS)
 
     int run_step(int mode, const void *src, void *dst, void *fout)
     {

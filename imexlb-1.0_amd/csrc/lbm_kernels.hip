@@ -722,7 +722,7 @@ __device__ __forceinline__ void step_small(const S *__restrict__ src, S *__restr
 template <int MODE>
 __host__ __device__ constexpr int step_cpt()
 {
-    return (MODE & K_COLLIDE) && !(MODE & (K_WRITE_F | K_WRITE_PU)) ? IMEXLB_STEP_CPT : 1;
+    return (MODE & K_COLLIDE) && !(MODE & (K_WRITE_F | K_WRITE_PU | K_SMALL)) ? IMEXLB_STEP_CPT : 1;
 }
 
 template <int D, class S, class C, int MODE>

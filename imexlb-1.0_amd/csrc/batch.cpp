@@ -2,7 +2,9 @@
 // size and parameters advanced together.  Each case keeps the single-lattice
 // layout and boundary tables (built by a host-only context); the cases are
 // concatenated in one set of device arrays, so one boundary-fill launch and
-// one step launch advance all of them.  The step kernel takes the cases'
+// one step launch advance all of them; the step writes the cases' boundary
+// copies itself (each case's push table), so after the first step only the
+// cases whose table does not qualify need the fill.  The step kernel takes the cases'
 // nodes in tiles of block*cells_per_thread nodes, by block index or from a
 // queue (persistent blocks); imexlb_batch_tune times the configurations and
 // keeps the fastest.  Pairs of steps replay as a HIP graph.
@@ -35,6 +37,12 @@ struct imexlb_batch {
     void *fill_dst = nullptr, *fill_src = nullptr;  // int32 (fill32) or int64 slots
     bool fill32 = false;
     int64_t nfill = 0;
+    // the fill's first nfill_step entries belong to cases without pushed
+    // copies and run every step; all nfill run before the first step
+    int64_t nfill_step = 0;
+    bool need_fill = true;
+    int32_t *push_beg = nullptr, *push_dst = nullptr;  // every case's push table, concatenated
+    uint8_t *push_q = nullptr;
     int64_t *link_out = nullptr, *link_back = nullptr;
     int8_t *link_q = nullptr;
     double *force_partial = nullptr, *force_out = nullptr;
@@ -77,7 +85,8 @@ struct imexlb_batch {
         drop_graphs();
         void *ptrs[] = {fb[0],     fb[1],      f,        p,         ux,        uy,            rv,
                         geo,       fill_dst,   fill_src, link_out,  link_back, link_q,        force_partial,
-                        force_out, mon,        d_cases,  queue,     tile_case, tile_first};
+                        force_out, mon,        d_cases,  queue,     tile_case, tile_first,
+                        push_beg,  push_dst,   push_q};
         for (void *q : ptrs)
             if (q) (void)hipFree(q);
         if (stream) (void)hipStreamDestroy(stream);
@@ -124,18 +133,25 @@ struct imexlb_batch {
         return IMEXLB_OK;
     }
 
-    hipError_t enqueue_step(int b)
+    // one step from buffer b: the boundary fill (every case's entries when
+    // full, else those of the cases without pushed copies; with persistent
+    // blocks the fill launch also rewinds their tile queue), then the step
+    hipError_t enqueue_step(int b, bool full)
     {
-        hipError_t e = launch_batch_fill(prec, fb[b], fb[1 - b], fill_dst, fill_src, fill32, nfill, queue, stream);
-        if (e != hipSuccess) return e;
+        const int64_t n = full ? nfill : nfill_step;
+        if (n > 0 || persist) {
+            hipError_t e = launch_batch_fill(prec, fb[b], fb[1 - b], fill_dst, fill_src, fill32, n, queue, stream);
+            if (e != hipSuccess) return e;
+        }
         return launch_batch_step(prec, block, cpt, persist, persist_grid, fb[b], fb[1 - b], geo, d_cases, tile_case,
                                  tile_first, ntiles, queue, stream);
     }
 
     int one_step()
     {
-        hipError_t e = enqueue_step(cur);
+        hipError_t e = enqueue_step(cur, need_fill);
         if (e != hipSuccess) return fail(IMEXLB_E_HIP, std::string("batch step: ") + hipGetErrorString(e));
+        need_fill = false;
         lazy = cur;
         cur = 1 - cur;
         have_pu = false;
@@ -152,8 +168,8 @@ struct imexlb_batch {
         HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = hipSuccess;
         for (int k = 0; k < kGraphPairs[size] && e == hipSuccess; ++k) {
-            e = enqueue_step(cur);
-            if (e == hipSuccess) e = enqueue_step(1 - cur);
+            e = enqueue_step(cur, false);
+            if (e == hipSuccess) e = enqueue_step(1 - cur, false);
         }
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(stream, &g);
@@ -238,8 +254,14 @@ int imexlb_batch_create_shapes(imexlb_batch **out, int ncase, const int32_t *nxn
     b->device = device;
     b->ssz = precision == 0 ? 8 : precision == 1 ? 4 : 2;
     std::vector<int64_t> fd, fs, lo, lbk;
+    std::vector<int64_t> fd_push, fs_push;  // fill entries of the cases with pushed copies (first step only)
+    std::vector<int32_t> pbeg, pdst;        // push tables of those cases, concatenated
+    std::vector<uint8_t> pq;
+    std::vector<int64_t> beg_off, ent_off;  // per case (-1: the case is filled every step)
     std::vector<int8_t> lq;
     std::vector<uint8_t> geo;
+    const char *pv = std::getenv("IMEXLB_PUSH");
+    const bool push_enabled = !(pv && pv[0] == '0');
     for (int i = 0; i < ncase; ++i) {
         imexlb_ctx *c = nullptr;
         int rc = imexlb_create(&c, 2, nxny[2 * i], nxny[2 * i + 1], 0, -1, precision);
@@ -260,10 +282,23 @@ int imexlb_batch_create_shapes(imexlb_batch **out, int ncase, const int32_t *nxn
         bc.geo_base = b->cell_total;
         b->cases.push_back(bc);
         const int64_t pb = bc.pop_base;
+        const bool push = push_enabled && t.push_ok && fill_fits_32(int64_t(c->Q) * c->plane);
+        std::vector<int64_t> &fdc = push ? fd_push : fd, &fsc = push ? fs_push : fs;
         for (size_t k = 0; k < t.fill_dst.size(); ++k) {
             const int64_t s = t.fill_src[k];
-            fd.push_back(t.fill_dst[k] + pb);
-            fs.push_back(s >= 0 ? s + pb : s == -1 ? -1 : encode_prev(decode_prev(s) + pb));
+            fdc.push_back(t.fill_dst[k] + pb);
+            fsc.push_back(s >= 0 ? s + pb : s == -1 ? -1 : encode_prev(decode_prev(s) + pb));
+        }
+        if (push) {  // slots within the case: the step offsets dst by pop_base
+            beg_off.push_back(int64_t(pbeg.size()));
+            ent_off.push_back(int64_t(pq.size()));
+            pbeg.insert(pbeg.end(), t.push_beg.begin(), t.push_beg.end());
+            pq.insert(pq.end(), t.push_q.begin(), t.push_q.end());
+            for (int64_t d : t.push_dst) pdst.push_back(int32_t(d));
+        } else {
+            beg_off.push_back(-1);
+            ent_off.push_back(-1);
+            for (uint8_t &g : t.geo) g = uint8_t(g & ~G_PUSH);
         }
         b->link_base.push_back(int64_t(lo.size()));
         b->link_count.push_back(int64_t(t.link_out.size()));
@@ -278,6 +313,9 @@ int imexlb_batch_create_shapes(imexlb_batch **out, int ncase, const int32_t *nxn
         b->cell_total += c->plane;
         b->node_total += int64_t(c->kp.N);
     }
+    b->nfill_step = int64_t(fd.size());
+    fd.insert(fd.end(), fd_push.begin(), fd_push.end());
+    fs.insert(fs.end(), fs_push.begin(), fs_push.end());
     HIP_TRY(hipSetDevice(device));
     const char *gv = std::getenv("IMEXLB_GRAPHS");
     b->graphs = !(gv && gv[0] == '0');
@@ -304,8 +342,22 @@ int imexlb_batch_create_shapes(imexlb_batch **out, int ncase, const int32_t *nxn
         (rc = balloc(B, (void **)&B->force_out, size_t(3 * ncase) * sizeof(double))) ||
         (rc = balloc(B, (void **)&B->mon, size_t(2 * ncase) * sizeof(unsigned long long))) ||
         (rc = balloc(B, (void **)&B->d_cases, size_t(ncase) * sizeof(BatchCase))) ||
-        (rc = balloc(B, (void **)&B->queue, 64)))
+        (rc = balloc(B, (void **)&B->queue, 64)) ||
+        (rc = balloc(B, (void **)&B->push_beg, std::max<size_t>(pbeg.size(), 1) * sizeof(int32_t))) ||
+        (rc = balloc(B, (void **)&B->push_dst, std::max<size_t>(pdst.size(), 1) * sizeof(int32_t))) ||
+        (rc = balloc(B, (void **)&B->push_q, std::max<size_t>(pq.size(), 1))))
         return rc;
+    if (!pbeg.empty()) HIP_TRY(hipMemcpy(B->push_beg, pbeg.data(), pbeg.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!pdst.empty()) HIP_TRY(hipMemcpy(B->push_dst, pdst.data(), pdst.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!pq.empty()) HIP_TRY(hipMemcpy(B->push_q, pq.data(), pq.size(), hipMemcpyHostToDevice));
+    for (int i = 0; i < ncase; ++i) {
+        PushTable &pt = B->cases[size_t(i)].pt;
+        pt = PushTable{};
+        if (beg_off[size_t(i)] < 0) continue;
+        pt.beg = B->push_beg + beg_off[size_t(i)];
+        pt.q = B->push_q + ent_off[size_t(i)];
+        pt.dst = B->push_dst + ent_off[size_t(i)];
+    }
     HIP_TRY(hipMemcpy(B->geo, geo.data(), geo.size(), hipMemcpyHostToDevice));
     HIP_TRY(upload_slots(B->fill_dst, fd, B->fill32));
     HIP_TRY(upload_slots(B->fill_src, fs, B->fill32));
@@ -350,6 +402,7 @@ static int batch_start(imexlb_batch *b, const double *r_vec)
     b->lazy = -1;
     b->have_pu = true;
     b->initialized = true;
+    b->need_fill = true;  // the first Collision wrote no boundary copies
     return IMEXLB_OK;
 }
 
@@ -436,6 +489,11 @@ int imexlb_batch_advance(imexlb_batch *b, int64_t nsteps)
     CHECK_BATCH(b);
     if (!b->initialized) return fail(IMEXLB_E_STATE, "imexlb_batch_init first");
     if (nsteps < 0) return fail(IMEXLB_E_ARG, "negative step count");
+    if (b->need_fill && nsteps > 0) {  // the first step after the start, with every case's fill
+        int rc = b->one_step();
+        if (rc) return rc;
+        --nsteps;
+    }
     if (!b->graphs) {
         for (int64_t k = 0; k < nsteps; ++k) {
             int rc = b->one_step();

@@ -120,6 +120,7 @@ struct BatchCase {
     KParams kp;
     int64_t pop_base;  // first population slot of the case
     int64_t geo_base;  // first cell of the case (node types, p, u)
+    PushTable pt;      // the case's boundary copies (int32 slots within the case), pushed by the step
 };
 
 struct StepArgs {

@@ -779,7 +779,9 @@ __global__ IMEXLB_STEP_BOUNDS void k_step(const S *__restrict__ src, S *__restri
 // interior nodes of every case are cut into tiles of BLOCK*CPT nodes; a
 // block takes a tile (by its index, or from a queue when PERSIST), reads the
 // tile's case and runs the fused pull on CPT nodes per thread, strided by
-// BLOCK.  Every case keeps its own parameters, populations and node types.
+// BLOCK.  Every case keeps its own parameters, populations and node types,
+// and the step writes its boundary copies from the case's own push table
+// (as the single-lattice step does) unless the case has none.
 template <int D, class S, class C, int BLOCK, int CPT, int PERSIST>
 __global__ __launch_bounds__(BLOCK) void k_batch_step(const S *__restrict__ src, S *__restrict__ dst,
                                                       const uint8_t *__restrict__ geo,
@@ -805,9 +807,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch_step(const S *__restrict__ src,
             const uint32_t i = first + uint32_t(k * BLOCK) + threadIdx.x;
             int x, y, z;
             uint32_t c;
-            if (node_of<D>(bc.kp, i, x, y, z, c))
-                step_node<D, S, C, K_COLLIDE>(src + bc.pop_base, dst + bc.pop_base, nullptr, nullptr, nullptr, nullptr,
-                                              nullptr, geo + bc.geo_base, bc.kp, x, y, z, c);
+            if (node_of<D>(bc.kp, i, x, y, z, c))  // cells of a case without pushed copies have no G_PUSH bit
+                step_node<D, S, C, K_COLLIDE | K_PUSH>(src + bc.pop_base, dst + bc.pop_base, nullptr, nullptr,
+                                                       nullptr, nullptr, nullptr, geo + bc.geo_base, bc.kp, x, y, z,
+                                                       c, bc.pt);
         }
         if (!PERSIST) return;
     }

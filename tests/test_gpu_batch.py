@@ -52,6 +52,29 @@ def test_batch_matches_single_lattices(block, cpt, persist, r_vec):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("nsteps", [1, 2, 37])
+def test_batch_fill_path_matches_pushed_copies(nsteps, r_vec, monkeypatch):
+    """The step kernel writes each case's boundary copies from its push
+    table; with IMEXLB_PUSH=0 the batch runs the boundary fill before every
+    step instead.  Both give the same fields and forces, bit for bit, also
+    for a first call of one or two steps (the step after the start still
+    takes every case's fill)."""
+    decks = _small_cases()
+    runs = []
+    for push in ("1", "0"):
+        monkeypatch.setenv("IMEXLB_PUSH", push)
+        b = Batch(decks, precision=I.F64)
+        b.init(r_vec)
+        b.advance(nsteps)
+        b.advance(3)
+        runs.append(([b.download_fields(i) for i in range(len(decks))], b.forces()))
+        b.close()
+    for (p0, u0), (p1, u1) in zip(runs[0][0], runs[1][0]):
+        assert np.array_equal(p0, p1) and np.array_equal(u0, u1)
+    assert np.array_equal(runs[0][1], runs[1][1])
+
+
+@pytest.mark.gpu
 def test_batch_64bit_slot_tables(r_vec, monkeypatch):
     """The batch's boundary fill with int64 slot tables (IMEXLB_SLOTS64=1,
     the form an ensemble above 2^31 population slots takes) gives the same

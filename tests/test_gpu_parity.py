@@ -126,6 +126,30 @@ def test_small_lattice_mode_does_not_change_results(small_cells, r_vec, monkeypa
         o.close()
 
 
+@pytest.mark.parametrize("nx,ny,precision", [(2048, 512, I.F64), (4096, 1024, I.F16_F32)])
+def test_midsize_lattices_small_form_bitwise(nx, ny, precision, r_vec, monkeypatch):
+    """2D lattices whose population buffers fit in the 256 MB Infinity Cache
+    (1.06 M cells in FP64, 4.2 M in FP16 storage) step with the small-lattice
+    form by default; forced off (IMEXLB_SMALL_CELLS=0), the other form gives
+    the same populations and fields, bit for bit."""
+    dk = I.parse_deck(f"{nx} {ny}\n0.05 0.2 100\n1.8867924528301887 1 1\n0.01 0.0025001\n", 2)
+    runs = []
+    for small in (None, "0"):
+        print(f"[midsize] {nx}x{ny} precision={precision} IMEXLB_SMALL_CELLS={small}", flush=True)
+        if small is None:
+            monkeypatch.delenv("IMEXLB_SMALL_CELLS", raising=False)
+        else:
+            monkeypatch.setenv("IMEXLB_SMALL_CELLS", small)
+        lat, _ = I.setup(dk, r_vec, device=0, precision=precision)
+        lat.advance(21)
+        runs.append((lat.download_pdf(I.PDF_F), lat.download_fields()))
+        lat.close()
+    (f0, (p0, u0)), (f1, (p1, u1)) = runs
+    assert np.isfinite(f0).all()
+    assert np.array_equal(f0, f1)
+    assert np.array_equal(p0, p1) and np.array_equal(u0, u1)
+
+
 @pytest.mark.parametrize("push", ["1", "0"])
 def test_64bit_slot_tables_match_oracle(push, r_vec, monkeypatch):
     """Lattices above 2^31 population slots keep their boundary tables in

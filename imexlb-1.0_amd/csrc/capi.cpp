@@ -259,7 +259,7 @@ int imexlb_create(imexlb_ctx **out, int dim, int nx, int ny, int nz, int device,
     if (dim != 2 && dim != 3) return fail(IMEXLB_E_ARG, "dim must be 2 or 3");
     if (nx < 2 || ny < 2 || (dim == 3 && nz < 2))
         return fail(IMEXLB_E_ARG, "every axis needs at least 3 nodes (cart_mpi.f90:107-110)");
-    if (precision < IMEXLB_F64 || precision > IMEXLB_BF16_F32) return fail(IMEXLB_E_ARG, "unknown precision");
+    if (precision < IMEXLB_F64 || precision > IMEXLB_BF16SR_F32) return fail(IMEXLB_E_ARG, "unknown precision");
     std::unique_ptr<imexlb_ctx> c(new (std::nothrow) imexlb_ctx);
     if (!c) return fail(IMEXLB_E_OOM, "context");
     c->dim = dim;

@@ -79,37 +79,75 @@ struct Lat {
 struct bf16_t {
     uint16_t bits;
 };
+// BF16 storage with stochastic rounding (IMEXLB_BF16SR_F32)
+struct bf16sr_t {
+    uint16_t bits;
+};
 
+// st(v, salt): the stored form of v; salt (the slot's cell and direction)
+// only feeds the stochastic rounding
 template <class S>
 struct Store;
 template <>
 struct Store<double> {
     __device__ static double ld(double v) { return v; }
     template <class C>
-    __device__ static double st(C v) { return double(v); }
+    __device__ static double st(C v, uint32_t = 0u) { return double(v); }
 };
 template <>
 struct Store<float> {
     __device__ static float ld(float v) { return v; }
     template <class C>
-    __device__ static float st(C v) { return float(v); }
+    __device__ static float st(C v, uint32_t = 0u) { return float(v); }
 };
 template <>
 struct Store<_Float16> {
     __device__ static float ld(_Float16 v) { return float(v); }
     template <class C>
-    __device__ static _Float16 st(C v) { return _Float16(float(v)); }  // round to nearest even
+    __device__ static _Float16 st(C v, uint32_t = 0u) { return _Float16(float(v)); }  // round to nearest even
 };
 template <>
 struct Store<bf16_t> {
     __device__ static float ld(bf16_t v) { return __uint_as_float(uint32_t(v.bits) << 16); }
     template <class C>
-    __device__ static bf16_t st(C v)
+    __device__ static bf16_t st(C v, uint32_t = 0u)
     {
         uint32_t u = __float_as_uint(float(v));
         if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return bf16_t{uint16_t(0x7fc0)};
         u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
         return bf16_t{uint16_t(u >> 16)};
+    }
+};
+
+// 32-bit integer mix (C. Wellons' published "lowbias32")
+__device__ __forceinline__ uint32_t mix32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    h *= 0x846ca68bu;
+    h ^= h >> 16;
+    return h;
+}
+
+// Stochastic rounding to BF16: the FP32 value's low 16 bits are compared
+// with a dither hashed from its bits and the salt, so it rounds away from
+// zero with probability equal to the dropped fraction and the rounding adds
+// no bias (a mean error of zero, against round-to-nearest's drift on the
+// slowly varying populations).  The same value at the same slot always
+// rounds the same way: runs repeat bit for bit.
+template <>
+struct Store<bf16sr_t> {
+    __device__ static float ld(bf16sr_t v) { return __uint_as_float(uint32_t(v.bits) << 16); }
+    template <class C>
+    __device__ static bf16sr_t st(C v, uint32_t salt = 0u)
+    {
+        const uint32_t u = __float_as_uint(float(v));
+        if ((u & 0x7f800000u) == 0x7f800000u)  // NaN, infinity
+            return bf16sr_t{uint16_t((u & 0x007fffffu) ? 0x7fc0u : (u >> 16))};
+        const uint32_t w = u + (mix32(u ^ (salt * 0x9e3779b9u)) & 0xffffu);
+        if ((w & 0x7f800000u) == 0x7f800000u) return bf16sr_t{uint16_t((u >> 16) | 0x7f7fu)};  // largest finite
+        return bf16sr_t{uint16_t(w >> 16)};
     }
 };
 
@@ -124,6 +162,11 @@ __device__ __forceinline__ void nt_store<bf16_t>(bf16_t *p, bf16_t v)
 {
     __builtin_nontemporal_store(v.bits, &p->bits);
 }
+template <>
+__device__ __forceinline__ void nt_store<bf16sr_t>(bf16sr_t *p, bf16sr_t v)
+{
+    __builtin_nontemporal_store(v.bits, &p->bits);
+}
 template <class S>
 __device__ __forceinline__ S nt_load(const S *p)
 {
@@ -133,6 +176,11 @@ template <>
 __device__ __forceinline__ bf16_t nt_load<bf16_t>(const bf16_t *p)
 {
     return bf16_t{__builtin_nontemporal_load(&p->bits)};
+}
+template <>
+__device__ __forceinline__ bf16sr_t nt_load<bf16sr_t>(const bf16sr_t *p)
+{
+    return bf16sr_t{__builtin_nontemporal_load(&p->bits)};
 }
 
 // ---------------------------------------------------------------------------
@@ -614,7 +662,7 @@ __device__ __forceinline__ void finish_node(const C (&f)[Lat<D>::Q], uint8_t g, 
         S out[kRegPush ? L::Q : 1];  // the stored values, for the copies
         collide<D, C>(f, p, ux, uy, uz, kp, [&](int q, C v) {
             S *o = (MODE & K_WAVE) ? wdst + (uint32_t(store_q(q, L::Q)) * 64u + (c & 63u)) : dst + cslot<L::Q>(kp, q, c);
-            const S sv = Store<S>::template st<C>(v);
+            const S sv = Store<S>::template st<C>(v, c * uint32_t(L::Q) + uint32_t(q));
             if constexpr (kRegPush) out[q] = sv;
             if (IMEXLB_STEP_NT)
                 nt_store(o, sv);
@@ -691,7 +739,7 @@ __device__ __forceinline__ void step_small(const S *__restrict__ src, S *__restr
             C p, ux, uy, uz;
             moments<D, C>(f, kp, x, y, z, p, ux, uy, uz);
             collide<D, C>(f, p, ux, uy, uz, kp, [&](int q, C v) {
-                const S sv = Store<S>::template st<C>(v);
+                const S sv = Store<S>::template st<C>(v, c * uint32_t(L::Q) + uint32_t(q));
                 dst[cslot<L::Q>(kp, q, c)] = sv;
                 if (eb < ee) sh[q * 64 + int(lane)] = __builtin_bit_cast(U, sv);
             });
@@ -848,7 +896,7 @@ __global__ __launch_bounds__(256) void k_collide(const S *__restrict__ fin, S *_
 #pragma unroll
     for (int q = 0; q < L::Q; ++q) f[q] = Store<S>::ld(fin[cslot<L::Q>(kp, q, c)]);
     collide<D, C>(f, C(p[c]), C(ux[c]), C(uy[c]), D == 3 ? C(uz[c]) : C(0), kp,
-                  [&](int q, C v) { dst[cslot<L::Q>(kp, q, c)] = Store<S>::template st<C>(v); });
+                  [&](int q, C v) { dst[cslot<L::Q>(kp, q, c)] = Store<S>::template st<C>(v, c * uint32_t(L::Q) + uint32_t(q)); });
 }
 
 // InitPDF (lbm.f90:29-47): f = feq(p, u) on fluid nodes.
@@ -885,7 +933,7 @@ __global__ __launch_bounds__(256) void k_init_pdf(S *__restrict__ fout, const do
         term(L::ey(q), u1);
         if (D == 3) term(L::ez(q), u2);
         const C feq = C(L::w(q)) * (pp * C(3) + rho0 * ((C(3) * edu + C(4.5) * (edu * edu)) - C(1.5) * udu));
-        fout[cslot<L::Q>(kp, q, c)] = Store<S>::template st<C>(feq);
+        fout[cslot<L::Q>(kp, q, c)] = Store<S>::template st<C>(feq, c * uint32_t(L::Q) + uint32_t(q));
     }
 }
 
@@ -1128,6 +1176,7 @@ static hipError_t dispatch_prec(int prec, F &&f)
     case 1: return f((float *)nullptr, (float *)nullptr);
     case 2: return f((_Float16 *)nullptr, (float *)nullptr);
     case 3: return f((bf16_t *)nullptr, (float *)nullptr);
+    case 4: return f((bf16sr_t *)nullptr, (float *)nullptr);
     default: return hipErrorInvalidValue;
     }
 }

@@ -58,7 +58,7 @@ contains
   end subroutine omp_set_default_device
 
   ! Storage/arithmetic form and phase execution from the environment
-  ! (IMEXLB_PRECISION = f64 | f32 | f16 | bf16, IMEXLB_FUSED = 1 | 0);
+  ! (IMEXLB_PRECISION = f64 | f32 | f16 | bf16 | bf16sr, IMEXLB_FUSED = 1 | 0);
   ! input.in keeps the reference's format.
   integer function imx_precision()
     character(len=16) :: v
@@ -73,6 +73,8 @@ contains
        imx_precision = IMEXLB_F16_F32
     case ('bf16')
        imx_precision = IMEXLB_BF16_F32
+    case ('bf16sr')
+       imx_precision = IMEXLB_BF16SR_F32
     end select
   end function imx_precision
 

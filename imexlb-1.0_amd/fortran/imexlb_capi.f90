@@ -6,7 +6,8 @@ module imexlb_capi
   implicit none
 
   integer(c_int), parameter :: IMEXLB_OK = 0
-  integer(c_int), parameter :: IMEXLB_F64 = 0, IMEXLB_F32 = 1, IMEXLB_F16_F32 = 2, IMEXLB_BF16_F32 = 3
+  integer(c_int), parameter :: IMEXLB_F64 = 0, IMEXLB_F32 = 1, IMEXLB_F16_F32 = 2, IMEXLB_BF16_F32 = 3, &
+                                IMEXLB_BF16SR_F32 = 4
   integer(c_int), parameter :: IMEXLB_LAYOUT_REF = 0, IMEXLB_LAYOUT_INTERIOR = 1
   integer(c_int), parameter :: IMEXLB_PDF_F = 0, IMEXLB_PDF_FB = 1
   integer(c_int), parameter :: IMEXLB_LIST_FLUID = 0, IMEXLB_LIST_BL = 1, IMEXLB_LIST_BR = 2, &

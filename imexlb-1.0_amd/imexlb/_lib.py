@@ -14,7 +14,7 @@ _LIB = None
 
 OK = 0
 E_ARG, E_HIP, E_OOM, E_STATE, E_UB, E_NODEV, E_NAN = -1, -2, -3, -4, -5, -6, -7
-F64, F32, F16_F32, BF16_F32 = 0, 1, 2, 3
+F64, F32, F16_F32, BF16_F32, BF16SR_F32 = 0, 1, 2, 3, 4
 LAYOUT_REF, LAYOUT_INTERIOR = 0, 1
 PDF_F, PDF_FB = 0, 1
 LISTS = {"fluid_id": 0, "bl": 1, "br": 2, "bu": 3, "bd": 4, "bf": 5, "bb": 6, "b_user": 7}

@@ -29,7 +29,7 @@ from .inputs import deck as get_deck
 from .inputs import derive, nodes
 from .lattice import setup
 
-FORMS = {"f64": _lib.F64, "f32": _lib.F32, "f16": _lib.F16_F32, "bf16": _lib.BF16_F32}
+FORMS = {"f64": _lib.F64, "f32": _lib.F32, "f16": _lib.F16_F32, "bf16": _lib.BF16_F32, "bf16sr": _lib.BF16SR_F32}
 
 
 def run_form(deck, prec, steps, every, r_vec):

@@ -1,7 +1,7 @@
 """Throughput of the time step on every workload and storage form.
 
     python -m imexlb.perf [--decks cyl400x100,sphere512x256x256,cyl8192x2048]
-                          [--forms f64,f32,f16_f32,bf16_f32] [--seconds 2.0]
+                          [--forms f64,f32,f16_f32,bf16_f32,bf16sr_f32] [--seconds 2.0]
                           [--no-graphs]
 
 For each (workload, form): set up the lattice on device 0, warm up, time
@@ -34,7 +34,8 @@ from .inputs import deck as get_deck, nodes as count_nodes
 from .lattice import setup
 
 HBM_PEAK_GBS = 8000.0
-FORMS = {"f64": (_lib.F64, 8), "f32": (_lib.F32, 4), "f16_f32": (_lib.F16_F32, 2), "bf16_f32": (_lib.BF16_F32, 2)}
+FORMS = {"f64": (_lib.F64, 8), "f32": (_lib.F32, 4), "f16_f32": (_lib.F16_F32, 2), "bf16_f32": (_lib.BF16_F32, 2),
+         "bf16sr_f32": (_lib.BF16SR_F32, 2)}
 
 
 def measure(deck_name, form, seconds=2.0, timed_steps=200, seed=2021):

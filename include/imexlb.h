@@ -55,6 +55,9 @@ typedef struct imexlb_ctx imexlb_ctx;
 #define IMEXLB_F32 1       /* FP32 storage, FP32 arithmetic                  */
 #define IMEXLB_F16_F32 2   /* FP16 storage of f, FP32 arithmetic              */
 #define IMEXLB_BF16_F32 3  /* BF16 storage of f, FP32 arithmetic              */
+#define IMEXLB_BF16SR_F32 4 /* BF16 storage of f with stochastic rounding (no
+                               rounding bias, more noise), FP32 arithmetic;
+                               deterministic: runs repeat bit for bit        */
 
 #define IMEXLB_LAYOUT_REF 0
 #define IMEXLB_LAYOUT_INTERIOR 1

@@ -149,7 +149,7 @@ def test_driver_matches_oracle(case, variant, env, r_vec, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec,rtol", [("f32", 1e-4), ("f16", 3e-2)])
+@pytest.mark.parametrize("prec,rtol", [("f32", 1e-4), ("f16", 3e-2), ("bf16sr", 1e-1)])
 def test_driver_storage_forms(prec, rtol, r_vec, tmp_path):
     """IMEXLB_PRECISION selects a storage form through the drop-in modules:
     the Monitor series of the 2D deck stays within rtol of the oracle's FP64

@@ -196,7 +196,8 @@ def main():
 
     variants = {}
     if not args.no_variants:
-        for name, prec, s in (("f32", I.F32, 4), ("f16_f32", I.F16_F32, 2), ("bf16_f32", I.BF16_F32, 2)):
+        for name, prec, s in (("f32", I.F32, 4), ("f16_f32", I.F16_F32, 2), ("bf16_f32", I.BF16_F32, 2),
+                              ("bf16sr_f32", I.BF16SR_F32, 2)):
             log(f"[bench] {DECK} {name}: {args.steps} timed steps")
             try:
                 v = gpu_leg(deck, r_vec, prec, args.steps, args.warmup)

@@ -84,8 +84,8 @@ struct bf16sr_t {
     uint16_t bits;
 };
 
-// st(v, salt): the stored form of v; salt (the slot's cell and direction)
-// only feeds the stochastic rounding
+// st(v, word): the stored form of v; word (a uniformly distributed 32-bit
+// value per slot and step, sr_word below) only feeds the stochastic rounding
 template <class S>
 struct Store;
 template <>
@@ -130,26 +130,41 @@ __device__ __forceinline__ uint32_t mix32(uint32_t h)
     return h;
 }
 
-// Stochastic rounding to BF16: the FP32 value's low 16 bits are compared
-// with a dither hashed from its bits and the salt, so it rounds away from
-// zero with probability equal to the dropped fraction and the rounding adds
-// no bias (a mean error of zero, against round-to-nearest's drift on the
-// slowly varying populations).  The same value at the same slot always
-// rounds the same way: runs repeat bit for bit.
+// Stochastic rounding to BF16: the FP32 value's low 16 bits are added to
+// the dither (the high half of word), so it rounds away from zero with
+// probability equal to the dropped fraction and the rounding adds no bias
+// (a mean error of zero, against round-to-nearest's drift on the slowly
+// varying populations).
 template <>
 struct Store<bf16sr_t> {
     __device__ static float ld(bf16sr_t v) { return __uint_as_float(uint32_t(v.bits) << 16); }
     template <class C>
-    __device__ static bf16sr_t st(C v, uint32_t salt = 0u)
+    __device__ static bf16sr_t st(C v, uint32_t word = 0u)
     {
         const uint32_t u = __float_as_uint(float(v));
         if ((u & 0x7f800000u) == 0x7f800000u)  // NaN, infinity
             return bf16sr_t{uint16_t((u & 0x007fffffu) ? 0x7fc0u : (u >> 16))};
-        const uint32_t w = u + (mix32(u ^ (salt * 0x9e3779b9u)) & 0xffffu);
+        const uint32_t w = u + (word >> 16);
         if ((w & 0x7f800000u) == 0x7f800000u) return bf16sr_t{uint16_t((u >> 16) | 0x7f7fu)};  // largest finite
         return bf16sr_t{uint16_t(w >> 16)};
     }
 };
+
+// The rounding words of a cell's Q stored values, sr_word(...) + q * kGolden:
+// one hash per cell and step of the cell's index and its pressure (which
+// changes every step), then a golden-ratio (Weyl) sequence over the
+// directions, so each value's dither is uniform and the Q dithers of a cell
+// are evenly spread.  Deterministic: runs repeat bit for bit.  Zero (and no
+// work) for every other storage form.
+constexpr uint32_t kGolden = 0x9e3779b9u;
+template <class S, class C>
+__device__ __forceinline__ uint32_t sr_word(C p, uint32_t c)
+{
+    if constexpr (std::is_same<S, bf16sr_t>::value)
+        return mix32(__float_as_uint(float(p)) ^ (c * kGolden));
+    else
+        return 0u;
+}
 
 // streaming (non-temporal) store of one population value
 template <class S>
@@ -660,9 +675,10 @@ __device__ __forceinline__ void finish_node(const C (&f)[Lat<D>::Q], uint8_t g, 
         // base plus the lane
         S *const wdst = (MODE & K_WAVE) ? dst + uint64_t(wave_tile(c)) * (L::Q * 64u) : dst;
         S out[kRegPush ? L::Q : 1];  // the stored values, for the copies
+        const uint32_t srw = sr_word<S>(p, c);
         collide<D, C>(f, p, ux, uy, uz, kp, [&](int q, C v) {
             S *o = (MODE & K_WAVE) ? wdst + (uint32_t(store_q(q, L::Q)) * 64u + (c & 63u)) : dst + cslot<L::Q>(kp, q, c);
-            const S sv = Store<S>::template st<C>(v, c * uint32_t(L::Q) + uint32_t(q));
+            const S sv = Store<S>::template st<C>(v, srw + uint32_t(q) * kGolden);
             if constexpr (kRegPush) out[q] = sv;
             if (IMEXLB_STEP_NT)
                 nt_store(o, sv);
@@ -738,8 +754,9 @@ __device__ __forceinline__ void step_small(const S *__restrict__ src, S *__restr
         if (g & G_FLUID) {
             C p, ux, uy, uz;
             moments<D, C>(f, kp, x, y, z, p, ux, uy, uz);
+            const uint32_t srw = sr_word<S>(p, c);
             collide<D, C>(f, p, ux, uy, uz, kp, [&](int q, C v) {
-                const S sv = Store<S>::template st<C>(v, c * uint32_t(L::Q) + uint32_t(q));
+                const S sv = Store<S>::template st<C>(v, srw + uint32_t(q) * kGolden);
                 dst[cslot<L::Q>(kp, q, c)] = sv;
                 if (eb < ee) sh[q * 64 + int(lane)] = __builtin_bit_cast(U, sv);
             });
@@ -896,7 +913,9 @@ __global__ __launch_bounds__(256) void k_collide(const S *__restrict__ fin, S *_
 #pragma unroll
     for (int q = 0; q < L::Q; ++q) f[q] = Store<S>::ld(fin[cslot<L::Q>(kp, q, c)]);
     collide<D, C>(f, C(p[c]), C(ux[c]), C(uy[c]), D == 3 ? C(uz[c]) : C(0), kp,
-                  [&](int q, C v) { dst[cslot<L::Q>(kp, q, c)] = Store<S>::template st<C>(v, c * uint32_t(L::Q) + uint32_t(q)); });
+                  [&, srw = sr_word<S>(C(p[c]), c)](int q, C v) {
+                      dst[cslot<L::Q>(kp, q, c)] = Store<S>::template st<C>(v, srw + uint32_t(q) * kGolden);
+                  });
 }
 
 // InitPDF (lbm.f90:29-47): f = feq(p, u) on fluid nodes.
@@ -933,7 +952,7 @@ __global__ __launch_bounds__(256) void k_init_pdf(S *__restrict__ fout, const do
         term(L::ey(q), u1);
         if (D == 3) term(L::ez(q), u2);
         const C feq = C(L::w(q)) * (pp * C(3) + rho0 * ((C(3) * edu + C(4.5) * (edu * edu)) - C(1.5) * udu));
-        fout[cslot<L::Q>(kp, q, c)] = Store<S>::template st<C>(feq, c * uint32_t(L::Q) + uint32_t(q));
+        fout[cslot<L::Q>(kp, q, c)] = Store<S>::template st<C>(feq, sr_word<S>(pp, c) + uint32_t(q) * kGolden);
     }
 }
 

@@ -457,9 +457,9 @@ def test_mass_matches_oracle(name, steps, fused, r_vec):
 
 # measured on MI355X (relative L2 error of u after 2,000 steps): FP32 1.2e-6,
 # FP16 storage 6.1e-4, BF16 storage 1.2e-2, BF16 with stochastic rounding
-# 6.7e-3; the bounds are three times those
+# 6.2e-3; the bounds are three times those
 @pytest.mark.parametrize("precision,tol", [(I.F32, 3.6e-6), (I.F16_F32, 1.8e-3), (I.BF16_F32, 3.7e-2),
-                                           (I.BF16SR_F32, 2.0e-2)])
+                                           (I.BF16SR_F32, 1.9e-2)])
 def test_precision_forms_error(precision, tol, r_vec):
     """FP32 arithmetic with FP32 / FP16 / BF16 / stochastically rounded BF16
     storage against the FP64 run of the 400x100 cylinder after 2000 steps:
@@ -481,7 +481,7 @@ def test_precision_forms_error(precision, tol, r_vec):
 def test_bf16_stochastic_rounding_repeats_and_dithers(r_vec):
     """BF16 storage with stochastic rounding: two runs of the 400x100
     cylinder give the same populations bit for bit (the dither is a hash of
-    the value and its slot), and they differ from round-to-nearest BF16."""
+    the cell and its pressure), and they differ from round-to-nearest BF16."""
     deck = get_deck("cyl400x100")
     runs = []
     for prec in (I.BF16SR_F32, I.BF16SR_F32, I.BF16_F32):

@@ -721,7 +721,8 @@ __device__ __forceinline__ void step_small(const S *__restrict__ src, S *__restr
                                            uint32_t tile)
 {
     static_assert(D == 2 && (MODE & K_PUSH) && !(MODE & K_IDX64), "K_SMALL: D2Q9 with int32 push slots");
-    static_assert(IMEXLB_CELLMAP && kSmallBlock == 64, "K_SMALL: one thread per padded cell, one-wave blocks");
+    static_assert(IMEXLB_CELLMAP && kSmallBlock == 64 && IMEXLB_TILE == 64,
+                  "K_SMALL: one thread per padded cell, one-wave blocks of one 64-cell tile");
     using L = Lat<D>;
     using U = BitsOf<S>;
     __shared__ U sh[L::Q * 64];
@@ -740,6 +741,10 @@ __device__ __forceinline__ void step_small(const S *__restrict__ src, S *__restr
             ds[r] = uint32_t(pdst[e]);
         }
     }
+    // the tile's slots from a wave-uniform base: slot q of lane l is
+    // base[store_q(q) * 64 + l] (cslot of cell c0 + l)
+    const S *const tsrc = src + uint64_t(tile) * (L::Q * 64u);
+    S *const tdst = dst + uint64_t(tile) * (L::Q * 64u);
     int x, y, z;
     uint32_t c;
     if (decode_cell<D>(kp, c0 + lane, x, y, z, c)) {
@@ -749,7 +754,7 @@ __device__ __forceinline__ void step_small(const S *__restrict__ src, S *__restr
         U own[L::Q];
         if (eb < ee) {  // what a body cell copies, loaded with the populations
 #pragma unroll
-            for (int q = 0; q < L::Q; ++q) own[q] = __builtin_bit_cast(U, src[cslot<L::Q>(kp, q, c)]);
+            for (int q = 0; q < L::Q; ++q) own[q] = __builtin_bit_cast(U, tsrc[uint32_t(store_q(q, L::Q)) * 64u + lane]);
         }
         if (g & G_FLUID) {
             C p, ux, uy, uz;
@@ -757,7 +762,7 @@ __device__ __forceinline__ void step_small(const S *__restrict__ src, S *__restr
             const uint32_t srw = sr_word<S>(p, c);
             collide<D, C>(f, p, ux, uy, uz, kp, [&](int q, C v) {
                 const S sv = Store<S>::template st<C>(v, srw + uint32_t(q) * kGolden);
-                dst[cslot<L::Q>(kp, q, c)] = sv;
+                tdst[uint32_t(store_q(q, L::Q)) * 64u + lane] = sv;
                 if (eb < ee) sh[q * 64 + int(lane)] = __builtin_bit_cast(U, sv);
             });
         } else if (g & G_PUSH) {  // (a cell with copies: eb < ee)

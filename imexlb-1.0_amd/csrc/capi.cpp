@@ -183,12 +183,12 @@ int build_device_tables(const imexlb_ctx *c, DeviceTables &t)
         t.push_beg.assign(size_t(c->kp.ncell) + 1, 0);
         t.push_q.resize(v.size());
         t.push_dst.resize(v.size());
-        t.push_lds.resize(c->dim == 2 ? v.size() : 0);
+        t.push_lds.resize(v.size());
         for (size_t k = 0; k < v.size(); ++k) {
             ++t.push_beg[size_t(v[k].cell) + 1];
             t.push_q[k] = v[k].q;
             t.push_dst[k] = v[k].dst;
-            if (c->dim == 2) t.push_lds[k] = uint16_t(v[k].q * 64u + (v[k].cell & 63u));
+            t.push_lds[k] = uint16_t(v[k].q * 64u + (v[k].cell & 63u));
             t.geo[v[k].cell] |= G_PUSH;
         }
         for (size_t k = 1; k < t.push_beg.size(); ++k) t.push_beg[k] += t.push_beg[k - 1];
@@ -479,7 +479,7 @@ int imexlb_set_geometry_shape(imexlb_ctx *c, double pos, double radius, double a
     const size_t np = std::max<size_t>(t.push_q.size(), 1);
     if ((rc = dev_alloc(c, (void **)&c->push_beg, t.push_beg.size() * sizeof(int32_t))) ||
         (rc = dev_alloc(c, (void **)&c->push_q, np)) || (rc = dev_alloc(c, &c->push_dst, np * isz)) ||
-        (c->dim == 2 && (rc = dev_alloc(c, (void **)&c->push_lds, np * sizeof(uint16_t)))))
+        (rc = dev_alloc(c, (void **)&c->push_lds, np * sizeof(uint16_t))))
         return rc;
     HIP_TRY(hipMemcpy(c->push_beg, t.push_beg.data(), t.push_beg.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     if (!t.push_q.empty()) HIP_TRY(hipMemcpy(c->push_q, t.push_q.data(), t.push_q.size(), hipMemcpyHostToDevice));

@@ -72,7 +72,7 @@ struct imexlb_ctx {
     int32_t *push_beg = nullptr;
     uint8_t *push_q = nullptr;
     void *push_dst = nullptr;   // int32 (fill32) or int64 slots
-    uint16_t *push_lds = nullptr;  // 2D: LDS position of each entry's value
+    uint16_t *push_lds = nullptr;  // LDS position of each entry's value
     bool push_enabled = true;   // IMEXLB_PUSH=0 keeps the separate fill
     bool push_on = false;       // enabled and the table qualifies
     bool prefilled = false;     // buffer cur already holds its boundary copies
@@ -323,7 +323,7 @@ struct DeviceTables {
     std::vector<int32_t> push_beg;  // ncell + 1 offsets
     std::vector<uint8_t> push_q;
     std::vector<int64_t> push_dst;
-    std::vector<uint16_t> push_lds;  // 2D lattices only
+    std::vector<uint16_t> push_lds;
     bool push_ok = true;  // every copy's source is a fluid node or a body cell
     std::vector<int64_t> link_out, link_back;
     std::vector<int8_t> link_q;

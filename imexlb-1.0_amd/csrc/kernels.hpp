@@ -109,8 +109,9 @@ struct PushTable {
     const int32_t *beg = nullptr;
     const uint8_t *q = nullptr;
     const void *dst = nullptr;  // int32 or int64 slots (K_IDX64)
-    // 2D lattices: q * 64 + (source cell % 64) of each entry, where the
-    // K_SMALL step finds the copied value in its wave's LDS
+    // q * 64 + (source cell % 64) of each entry, where the small-lattice
+    // step and the 16/32-bit D3Q27 step find the copied value in their
+    // wave's LDS
     const uint16_t *lds = nullptr;
 };
 

@@ -780,6 +780,72 @@ __device__ __forceinline__ void step_small(const S *__restrict__ src, S *__restr
     }
 }
 
+// The fused step of a D3Q27 lattice in 16- or 32-bit storage, one wave per
+// 64-cell tile [c0, c0 + 64): as step_small, each wave with boundary copies
+// leaves its cells' values in LDS (27 x 64 values) and spreads the tile's
+// run of the push table over its lanes, one LDS read and one store per copy,
+// instead of each cell reading back its own output per copy.  Output stores
+// non-temporal.  (FP64 keeps the read-back: 13.8 KB of LDS per wave would
+// cost the kernel a wave per SIMD.)
+template <int D, class S, class C, int MODE>
+__device__ __forceinline__ void step_wave_copies(const S *__restrict__ src, S *__restrict__ dst,
+                                                 const uint8_t *__restrict__ geo, const KParams &kp,
+                                                 const PushTable &pt, uint32_t c0, uint32_t lane, BitsOf<S> *sh)
+{
+    static_assert(IMEXLB_CELLMAP && IMEXLB_TILE == 64 && (MODE & K_PUSH) && !(MODE & K_IDX64),
+                  "one wave per 64-cell tile, int32 push slots");
+    using L = Lat<D>;
+    using U = BitsOf<S>;
+    if (c0 >= kp.ncell) return;  // the whole wave
+    const int32_t eb = __builtin_amdgcn_readfirstlane(pt.beg[c0]);
+    const int32_t ee = __builtin_amdgcn_readfirstlane(pt.beg[min(c0 + 64u, kp.ncell)]);
+    const int32_t *const pdst = static_cast<const int32_t *>(pt.dst);
+    uint32_t li[kSmallRounds] = {}, ds[kSmallRounds] = {};
+#pragma unroll
+    for (int r = 0; r < kSmallRounds; ++r) {
+        const int32_t e = eb + int32_t(lane) + 64 * r;
+        if (e < ee) {
+            li[r] = pt.lds[e];
+            ds[r] = uint32_t(pdst[e]);
+        }
+    }
+    int x, y, z;
+    uint32_t c;
+    if (decode_cell<D>(kp, c0 + lane, x, y, z, c)) {
+        const uint8_t g = geo[c];
+        C f[L::Q];
+        gather<D, S, C, MODE>(src, kp, c, f);
+        if (g & G_FLUID) {
+            C p, ux, uy, uz;
+            moments<D, C>(f, kp, x, y, z, p, ux, uy, uz);
+            const uint32_t srw = sr_word<S>(p, c);
+            collide<D, C>(f, p, ux, uy, uz, kp, [&](int q, C v) {
+                const S sv = Store<S>::template st<C>(v, srw + uint32_t(q) * kGolden);
+                S *const o = dst + cslot<L::Q>(kp, q, c);
+                if (IMEXLB_STEP_NT)
+                    nt_store(o, sv);
+                else
+                    *o = sv;
+                if (eb < ee) sh[q * 64 + int(lane)] = __builtin_bit_cast(U, sv);
+            });
+        } else if (g & G_PUSH) {  // a body cell: its previous-step values
+#pragma unroll
+            for (int q = 0; q < L::Q; ++q) sh[q * 64 + int(lane)] = __builtin_bit_cast(U, src[cslot<L::Q>(kp, q, c)]);
+        }
+    }
+    if (eb < ee) {  // the same for every lane of the wave
+        // the copied values are other lanes' LDS writes of this wave
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int r = 0; r < kSmallRounds; ++r)
+            if (eb + int32_t(lane) + 64 * r < ee) dst[ds[r]] = __builtin_bit_cast(S, sh[li[r]]);
+        for (int32_t e = eb + int32_t(lane) + 64 * kSmallRounds; e < ee; e += 64)
+            dst[uint32_t(pdst[e])] = __builtin_bit_cast(S, sh[pt.lds[e]]);
+    }
+}
+
 #if IMEXLB_STEP_WAVES
 #define IMEXLB_STEP_BOUNDS __launch_bounds__(IMEXLB_STEP_BLOCK, IMEXLB_STEP_WAVES)
 #else
@@ -806,6 +872,11 @@ __global__ IMEXLB_STEP_BOUNDS void k_step(const S *__restrict__ src, S *__restri
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, kp.xcd_remap);
     if constexpr ((MODE & K_SMALL) != 0) {
         step_small<D, S, C, MODE>(src, dst, geo, kp, pt, blk);
+    } else if constexpr (D == 3 && sizeof(S) <= 4 && MODE == (K_COLLIDE | K_PUSH) && CPT == 1 && IMEXLB_CELLMAP &&
+                         IMEXLB_TILE == 64) {
+        __shared__ BitsOf<S> sh[IMEXLB_STEP_BLOCK / 64][Lat<D>::Q * 64];
+        step_wave_copies<D, S, C, MODE>(src, dst, geo, kp, pt, blk * IMEXLB_STEP_BLOCK + (threadIdx.x & ~63u),
+                                        threadIdx.x & 63u, sh[threadIdx.x >> 6]);
     } else if constexpr (CPT == 1) {
         int x, y, z;
         uint32_t c;

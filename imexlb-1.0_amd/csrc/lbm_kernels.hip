@@ -106,16 +106,29 @@ struct Store<_Float16> {
     template <class C>
     __device__ static _Float16 st(C v, uint32_t = 0u) { return _Float16(float(v)); }  // round to nearest even
 };
+// two BF16 values in one register, the operand of the conversion builtins
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 template <>
 struct Store<bf16_t> {
     __device__ static float ld(bf16_t v) { return __uint_as_float(uint32_t(v.bits) << 16); }
+    // round to nearest even in one instruction (gfx950's v_cvt_pk_bf16_f32;
+    // a NaN stays a quiet NaN)
     template <class C>
     __device__ static bf16_t st(C v, uint32_t = 0u)
     {
-        uint32_t u = __float_as_uint(float(v));
-        if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return bf16_t{uint16_t(0x7fc0)};
-        u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-        return bf16_t{uint16_t(u >> 16)};
+        return bf16_t{__builtin_bit_cast(uint16_t, static_cast<__bf16>(float(v)))};
+    }
+    // The same rounding through the stochastic-rounding instruction, whose
+    // word is then the round-to-nearest-even increment 0x7fff + lsb in its
+    // high half: three instructions, equal to st() for every input but NaN
+    // payloads.  The D3Q27 step with LDS copies is scheduled 15 % slower
+    // around the one-instruction form.
+    template <class C>
+    __device__ static bf16_t st_via_sr(C v)
+    {
+        const float f = float(v);
+        const uint32_t w = (0x7fffu + ((__float_as_uint(f) >> 16) & 1u)) << 16;
+        return bf16_t{uint16_t(__builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_sr_bf16_f32(bf16x2_t{}, f, w, false)))};
     }
 };
 
@@ -131,22 +144,20 @@ __device__ __forceinline__ uint32_t mix32(uint32_t h)
 }
 
 // Stochastic rounding to BF16: the FP32 value's low 16 bits are added to
-// the dither (the high half of word), so it rounds away from zero with
-// probability equal to the dropped fraction and the rounding adds no bias
-// (a mean error of zero, against round-to-nearest's drift on the slowly
-// varying populations).
+// the dither (the high half of word) and the sum is truncated, so it rounds
+// away from zero with probability equal to the dropped fraction and the
+// rounding adds no bias (a mean error of zero, against round-to-nearest's
+// drift on the slowly varying populations).  One instruction, gfx950's
+// v_cvt_sr_bf16_f32, which is this sum for every finite input (checked
+// against the bit formula on 10^6 inputs); a finite value carried past the
+// largest BF16 becomes infinity, NaN stays NaN.
 template <>
 struct Store<bf16sr_t> {
     __device__ static float ld(bf16sr_t v) { return __uint_as_float(uint32_t(v.bits) << 16); }
     template <class C>
     __device__ static bf16sr_t st(C v, uint32_t word = 0u)
     {
-        const uint32_t u = __float_as_uint(float(v));
-        if ((u & 0x7f800000u) == 0x7f800000u)  // NaN, infinity
-            return bf16sr_t{uint16_t((u & 0x007fffffu) ? 0x7fc0u : (u >> 16))};
-        const uint32_t w = u + (word >> 16);
-        if ((w & 0x7f800000u) == 0x7f800000u) return bf16sr_t{uint16_t((u >> 16) | 0x7f7fu)};  // largest finite
-        return bf16sr_t{uint16_t(w >> 16)};
+        return bf16sr_t{uint16_t(__builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_sr_bf16_f32(bf16x2_t{}, float(v), word, false)))};
     }
 };
 
@@ -820,7 +831,11 @@ __device__ __forceinline__ void step_wave_copies(const S *__restrict__ src, S *_
             moments<D, C>(f, kp, x, y, z, p, ux, uy, uz);
             const uint32_t srw = sr_word<S>(p, c);
             collide<D, C>(f, p, ux, uy, uz, kp, [&](int q, C v) {
-                const S sv = Store<S>::template st<C>(v, srw + uint32_t(q) * kGolden);
+                S sv;
+                if constexpr (std::is_same<S, bf16_t>::value)
+                    sv = Store<S>::template st_via_sr<C>(v);
+                else
+                    sv = Store<S>::template st<C>(v, srw + uint32_t(q) * kGolden);
                 S *const o = dst + cslot<L::Q>(kp, q, c);
                 if (IMEXLB_STEP_NT)
                     nt_store(o, sv);

@@ -79,7 +79,7 @@ def measure(deck_name, form, seconds=2.0, timed_steps=200, seed=2021):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--decks", default="cyl400x100,sphere512x256x256,cyl8192x2048")
-    ap.add_argument("--forms", default="f64,f32,f16_f32,bf16_f32")
+    ap.add_argument("--forms", default="f64,f32,f16_f32,bf16_f32,bf16sr_f32")
     ap.add_argument("--seconds", type=float, default=2.0)
     ap.add_argument("--timed-steps", type=int, default=200)
     ap.add_argument("--no-graphs", action="store_true", help="launch every step on its own (IMEXLB_GRAPHS=0)")

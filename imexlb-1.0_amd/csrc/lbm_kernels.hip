@@ -820,6 +820,11 @@ __device__ __forceinline__ void step_wave_copies(const S *__restrict__ src, S *_
             ds[r] = uint32_t(pdst[e]);
         }
     }
+    // the tile's slots from a wave-uniform base: slot q of lane l is
+    // base[store_q(q) * 64 + l] (cslot of cell c0 + l)
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(c0 >> 6);
+    const S *const tsrc = src + uint64_t(tile) * (L::Q * 64u);
+    S *const tdst = dst + uint64_t(tile) * (L::Q * 64u);
     int x, y, z;
     uint32_t c;
     if (decode_cell<D>(kp, c0 + lane, x, y, z, c)) {
@@ -836,7 +841,7 @@ __device__ __forceinline__ void step_wave_copies(const S *__restrict__ src, S *_
                     sv = Store<S>::template st_via_sr<C>(v);
                 else
                     sv = Store<S>::template st<C>(v, srw + uint32_t(q) * kGolden);
-                S *const o = dst + cslot<L::Q>(kp, q, c);
+                S *const o = tdst + (uint32_t(store_q(q, L::Q)) * 64u + lane);
                 if (IMEXLB_STEP_NT)
                     nt_store(o, sv);
                 else
@@ -845,7 +850,8 @@ __device__ __forceinline__ void step_wave_copies(const S *__restrict__ src, S *_
             });
         } else if (g & G_PUSH) {  // a body cell: its previous-step values
 #pragma unroll
-            for (int q = 0; q < L::Q; ++q) sh[q * 64 + int(lane)] = __builtin_bit_cast(U, src[cslot<L::Q>(kp, q, c)]);
+            for (int q = 0; q < L::Q; ++q)
+                sh[q * 64 + int(lane)] = __builtin_bit_cast(U, tsrc[uint32_t(store_q(q, L::Q)) * 64u + lane]);
         }
     }
     if (eb < ee) {  // the same for every lane of the wave

@@ -908,6 +908,12 @@ __global__ IMEXLB_STEP_BOUNDS void k_step(const S *__restrict__ src, S *__restri
             C f[Lat<D>::Q];
             gather_wave<D, S, C>(src, kp, c, f);
             finish_node<D, S, C, MODE | K_WAVE>(f, g, src, dst, fout, pout, uxo, uyo, uzo, kp, x, y, z, c, pt);
+        } else if constexpr (IMEXLB_CELLMAP && IMEXLB_TILE == 64 && MODE == (MODE & (K_COLLIDE | K_PUSH | K_IDX64))) {
+            // the wave is one tile: output slots from its wave-uniform base
+            const uint8_t g = geo[c];
+            C f[Lat<D>::Q];
+            gather<D, S, C, MODE>(src, kp, c, f);
+            finish_node<D, S, C, MODE | K_WAVE>(f, g, src, dst, fout, pout, uxo, uyo, uzo, kp, x, y, z, c, pt);
         } else {
             step_node<D, S, C, MODE>(src, dst, fout, pout, uxo, uyo, uzo, geo, kp, x, y, z, c, pt);
         }

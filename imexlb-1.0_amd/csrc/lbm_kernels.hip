@@ -867,13 +867,15 @@ __device__ __forceinline__ void step_wave_copies(const S *__restrict__ src, S *_
     }
 }
 
-// Threads per block of the fused step: one wave for the small-lattice form,
-// 128 for D2Q9 in FP64 storage (2.5 % faster on the 8192x2048 cylinder than
-// 256; the 16-bit forms lose 2-4 % at 128), IMEXLB_STEP_BLOCK otherwise.
+// Threads per block of the fused step: one wave for the small-lattice form
+// and for D3Q27 in FP32 storage (2.8 % faster on the sphere than 256), 128
+// for D2Q9 in FP64 storage (2.5 % faster on the 8192x2048 cylinder), and
+// IMEXLB_STEP_BLOCK otherwise (the 16-bit forms lose 2-4 % at 128 and
+// 11-18 % at 64, FP64 D3Q27 0.4-0.6 % at 64 and 128).
 template <int D, class S, int MODE>
 __host__ __device__ constexpr unsigned step_block()
 {
-    return (MODE & K_SMALL) ? unsigned(kSmallBlock)
+    return (MODE & K_SMALL) || (D == 3 && sizeof(S) == 4) ? unsigned(kSmallBlock)
            : (D == 2 && sizeof(S) == 8 && IMEXLB_STEP_BLOCK >= 128) ? 128u
                                                                    : unsigned(IMEXLB_STEP_BLOCK);
 }
